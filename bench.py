@@ -1,0 +1,146 @@
+"""Benchmark of the fused D3Q19 step on one MI355X.
+
+Workload (default): the H=40 porous stone wall at 20 % void fraction, 640 x 320 x 160
+cells (32.8 M), gusting inlet, nu = 2e-4, Cs = 0.16, free-slip top and sides, FP32
+populations.  A step is one fused kernel launch over the whole lattice (plus, every 64
+steps, the FP64 force-row reduction).  Inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line: MLUPS over all nx*ny*nz cells (as lbm3d.py:384 counts them), the
+step kernel's roofline figure (algorithmic 152 B per FP32 cell update, i.e. 19 loads +
+19 stores of 4 B; 76 B for the 16-bit forms) against 8 TB/s, and a CPU baseline: the
+FP32 oracle (a C restatement of the reference kernel) timed on the host on a bounded
+sample of the same lattice.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "jeju-stonewall-lbm_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS and % HBM roofline (FP32 vs FP16-storage, Cd err vs FP64); p50/p99 case latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def traffic_from_profile(dtype):
+    """HBM bytes per step launch measured by rocprofv3 PMC counters, if a summary exists."""
+    p = os.path.join(ROOT, "profiles", f"k_step_porous_wall_640x320x160_{dtype}_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(w, budget_s=15.0, max_steps=20):
+    """FP32 oracle on the host: as many steps of the same lattice as fit in ~budget_s."""
+    import numpy as np
+    from oracle import c_oracle as co
+    threads = min(16, os.cpu_count() or 1)
+    co.set_threads(threads)
+    f = co.init(w.flag, w.uin, "f32")
+    g = np.empty_like(f)
+    yb = 1 if w.ybc == "freeslip" else 0
+    steps, t0 = 0, time.perf_counter()
+    while steps < max_steps:
+        co.step(f, w.flag, w.uin, w.nu0, w.cs, w.x_sponge, w.nu_sponge, gust=float(w.gust[steps % len(w.gust)]),
+                ybc=yb, f_out=g)
+        f, g = g, f
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    nx, ny, nz = w.flag.shape
+    return dict(value=steps * nx * ny * nz / el / 1e6, unit="MLUPS", cores=threads, kind="port",
+                sample=f"{steps} steps of the {nx}x{ny}x{nz} lattice, FP32 C oracle, {threads} OpenMP threads, "
+                       f"{el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--H", type=int, default=40)
+    ap.add_argument("--porosity", type=float, default=0.2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the stone-wall step runs on a single MI355X here "
+              "(multi-GPU x-slab sharding is a design note, not a measured path). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from stonewall3d import workloads
+    from stonewall3d.lbm3d import Simulation, bytes_per_update
+
+    n_total = a.warmup + a.steps
+    w = workloads.porous_wall(H=a.H, porosity=a.porosity, seed=0, n_steps=n_total)
+    nx, ny, nz = w.shape
+    N = nx * ny * nz
+    sim = Simulation(w.flag, w.uin, dtype=a.dtype, **w.sim_kwargs())
+    sim.init(noise=0.02, seed=0)
+    sim.step(a.warmup)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(a.steps):
+        ev[k][0].record(stream)
+        sim.step(1)
+        ev[k][1].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    launch_ms = [s.elapsed_time(e) for s, e in ev]
+    kern_ms = float(np.mean(launch_ms))
+    F = sim.force_rows()
+    if not np.isfinite(F).all():
+        raise SystemExit("force rows are not finite")
+    fluid = int((w.flag == 0).sum())
+    mlups = a.steps * N / (total_ms * 1e-3) / 1e6
+    bpu = bytes_per_update(a.dtype)
+    achieved = N * bpu / (kern_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile(a.dtype)
+    cd = sim.coefficients(w.UH, w.H)["Cd"]
+
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(total_ms / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.dtype,
+        "data": "synthetic (seeded rounded-box stone wall, seeded gust)",
+        "config": {"workload": f"porous stone wall H={a.H}, void {a.porosity}, {nx}x{ny}x{nz} cells, gusting inlet, "
+                               f"nu=2e-4, Cs=0.16, free-slip top/sides",
+                   "cells": N, "fluid_cells": fluid, "bytes_per_update": bpu},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_step", "kernel_avg_ms": round(kern_ms, 4)},
+        "fluid_mlups": round(a.steps * fluid / (total_ms * 1e-3) / 1e6, 1),
+        "mean_cd": float(cd[a.warmup:].mean()),
+    }
+    del sim
+    torch.cuda.empty_cache()
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

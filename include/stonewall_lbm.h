@@ -1,0 +1,113 @@
+/*
+ * stonewall_lbm.h -- C ABI of the MI355X (gfx950) D3Q19 stone-wall lattice-Boltzmann step.
+ *
+ * The reference (Cat-yejun/jeju-stonewall-lbm) drives its kernels through Warp's launch
+ * FFI, wp.launch(kernel, dim, inputs=[...]), from stonewall3d/lbm3d.py.  Each entry
+ * point below replaces one of those launches; the comment on each names the reference
+ * line it replaces.  The Python host surface (stonewall3d.lbm3d in this package) calls
+ * them through ctypes at the same places the reference calls wp.launch.
+ *
+ * Conventions
+ *   - Every function returns int: 0 = success, a value below 1000 = the hipError_t
+ *     reported by the runtime, 1001+ = a library error (SW_ERR_*).  sw_last_error()
+ *     returns a message for the calling thread's last failure.
+ *   - All pointers are device pointers unless stated.  The caller owns every buffer;
+ *     no entry point on the step path allocates or frees memory.
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Launches are asynchronous
+ *     and ordered on that stream; nothing synchronizes except sw_prepare_links.
+ *   - Populations are stored structure-of-arrays with x contiguous:
+ *       f[i*N + (z*ny + y)*nx + x],  N = nx*ny*nz,  i = 0..18 in the reference's
+ *     direction order (lbm3d.py:20-24).  sw_to_ref_layout / sw_from_ref_layout convert
+ *     to and from the reference's (19, nx, ny, nz) z-contiguous arrays.
+ *   - dtype: SW_F32 (float), SW_F16 / SW_BF16 (16-bit storage of (f_i - w_i) * scale,
+ *     FP32 arithmetic; scale a power of two), SW_F64 (double storage and arithmetic).
+ *   - A lattice holds at most 113,025,455 cells.
+ */
+#ifndef STONEWALL_LBM_H
+#define STONEWALL_LBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sw_dtype { SW_F32 = 0, SW_F16 = 1, SW_BF16 = 2, SW_F64 = 3 };
+enum sw_ybc { SW_Y_PERIODIC = 0, SW_Y_FREESLIP = 1 };
+enum sw_flag { SW_FLUID = 0, SW_WALL = 1, SW_GROUND = 2 };           /* lbm3d.py:30 */
+enum sw_error {
+    SW_OK = 0,
+    SW_ERR_INVALID = 1001,
+    SW_ERR_GEOMETRY = 1002,
+    SW_ERR_TOO_LARGE = 1003,
+    SW_ERR_NOT_FOUND = 1004,
+    SW_ERR_CAPACITY = 1005
+};
+
+/* Library version (major*100 + minor*10 + patch). */
+int sw_version(void);
+
+/* Message for the calling thread's last failed call. */
+const char* sw_last_error(void);
+
+/* Rows (y lines) per step workgroup; the partials ring holds ceil(ny/rows) pairs per z. */
+int sw_rows_per_block(void);
+
+/* Doubles per step in the force-partials ring: nz * ceil(ny/rows) * 2. */
+long long sw_partials_per_step(int ny, int nz);
+
+/* Builds the per-cell link masks from the solid flags.
+ * Replaces the flag set-up at lbm3d.py:288-290 as the kernels consume it: the
+ * reference's k_step tests the neighbour flag of every link on every step
+ * (lbm3d.py:88-91); here that test is done once.
+ *   flag  : (nx, ny, nz) uint8, z contiguous, values FLUID/WALL/GROUND as lbm3d.py:30;
+ *           the layer z=0 must be solid, and flag >= 2 may only be reached through
+ *           links with cz = +1 (the ground), else SW_ERR_GEOMETRY.
+ *   links : [nz][ny][nx] uint32 output.
+ *   ybc   : SW_Y_PERIODIC (the reference, lbm3d.py:79) or SW_Y_FREESLIP.
+ * Synchronizes `stream` (it reports geometry errors). */
+int sw_prepare_links(const uint8_t* flag, uint32_t* links, int nx, int ny, int nz, int ybc, void* stream);
+
+/* Writes the equilibrium f = feq(1, u) into f.  Replaces k_init, lbm3d.py:39-54,
+ * launched at lbm3d.py:340.  Fluid cells get u = (uin[z](1+noise r1), uin[z] noise r2,
+ * uin[z] noise r3) with r in [-1,1) from a PCG hash of (seed, cell); solid cells u = 0. */
+int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx, int ny, int nz,
+            float noise, unsigned int seed, float scale, void* stream);
+
+/* One fused step f_in -> f_out.  Replaces k_step, lbm3d.py:57-155, launched at
+ * lbm3d.py:359-361.
+ *   uin        : [nz] inlet profile (lbm3d.py:274-275); gust: per-step inlet factor
+ *                table indexed by t (NULL = no gust, the reference's steady inlet).
+ *   t, t_dev   : step index = t + *t_dev (t_dev may be NULL); selects the gust entry
+ *                and the partials slot t % ring.
+ *   nu0, cs, x_sponge, nu_sponge : as lbm3d.py:361 (nu_sponge = 0.05 there).
+ *   partials   : FP64 force-partials ring, ring * sw_partials_per_step(ny, nz) doubles,
+ *                or NULL to skip the force.  Reduce with sw_force_reduce.
+ * Solid cells of f_out are not written (as the reference, lbm3d.py:65-66). */
+int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
+            const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
+            int x_sponge, double nu_sponge, int ybc, float scale, double* partials, int ring,
+            void* stream);
+
+/* force[t][z][0..1] (FP64 Fx, Fz per height level) for t = t0 + *t_dev .. + nsteps - 1,
+ * summed in a fixed order from the partials ring.  Replaces the FP32 atomic scatter
+ * wp.atomic_add(force, t, z, c, ...) at lbm3d.py:99-101 (force array :301). */
+int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int ny, int nz,
+                    double* force, void* stream);
+
+/* (rho, ux, uy, uz) per cell into U[4][N] and/or Uacc[4][N] += that (either may be NULL).
+ * Replaces k_macro + k_accum, lbm3d.py:158-188, launched at lbm3d.py:367-369 and :387.
+ * Solid cells give (1, 0, 0, 0). */
+int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale, float* U,
+             float* Uacc, void* stream);
+
+/* Layout conversion of ncomp stacked fields between [c][z][y][x] (this library) and
+ * [c][x][y][z] (the reference's arrays, lbm3d.py:293-303, used by its checkpoint files,
+ * lbm3d.py:342-355).  elem_bytes in {1, 2, 4, 8}; src and dst must not overlap. */
+int sw_to_ref_layout(const void* src, void* dst, int elem_bytes, int ncomp, int nx, int ny, int nz, void* stream);
+int sw_from_ref_layout(const void* src, void* dst, int elem_bytes, int ncomp, int nx, int ny, int nz, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STONEWALL_LBM_H */

@@ -1,0 +1,152 @@
+// Shared device-side definitions for the D3Q19 stone-wall LBM kernels (gfx950).
+//
+// Lattice: D3Q19 in the reference's direction order (stonewall3d/lbm3d.py:20-27):
+//   C = (0,0,0), (+-1,0,0), (0,+-1,0), (0,0,+-1), (1,1,0),(-1,-1,0),(1,-1,0),(-1,1,0),
+//       (1,0,1),(-1,0,-1),(1,0,-1),(-1,0,1), (0,1,1),(0,-1,-1),(0,1,-1),(0,-1,1)
+//   W = 1/3, 1/18 x6, 1/36 x12 (rounded to float32 as numpy does, lbm3d.py:25)
+//
+// Memory layout in HBM (structure of arrays, x contiguous):
+//   population i of cell (x,y,z) lives at  f[i*N + (z*ny + y)*nx + x],  N = nx*ny*nz.
+//   A wavefront owns one (y,z) row and sweeps it in 64-cell chunks, so each of the
+//   19 loads and 19 stores of a chunk is one contiguous 64-element access.
+//
+// Link masks (one uint32 per cell, built once from the solid flags):
+//   bit i (i = 1..18) : the source cell of link i is solid -> half-way bounce-back
+//   bits 19..23       : for the five links with cz = +1 (i = 5,11,14,15,18), the solid
+//                       source is ground (flag >= 2): bounce back but no wall force
+//   bit 31            : the cell itself is solid (not updated)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <utility>
+
+namespace sw {
+
+constexpr int Q = 19;
+constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+constexpr int MIRZ[Q] = {0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16};
+constexpr int MIRY[Q] = {0, 1, 2, 4, 3, 5, 6, 9, 10, 7, 8, 11, 12, 13, 14, 18, 17, 16, 15};
+
+constexpr uint32_t SOLID_BIT = 1u << 31;
+constexpr int GROUND_BIT0 = 19;
+// index (0..4) of a cz=+1 link among {5,11,14,15,18}, -1 otherwise
+constexpr int GROUND_SLOT[Q] = {-1, -1, -1, -1, -1, 0, -1, -1, -1, -1, -1, 1, -1, -1, 2, 3, -1, -1, 4};
+
+enum YBC { Y_PERIODIC = 0, Y_FREESLIP = 1 };
+
+template <class T> __host__ __device__ constexpr T wgt(int i) {
+    return i == 0 ? T(1.0 / 3.0) : (i < 7 ? T(1.0 / 18.0) : T(1.0 / 36.0));
+}
+// float32 weights are the float rounding of the double fractions (numpy float32 cast)
+template <> __host__ __device__ constexpr float wgt<float>(int i) {
+    return i == 0 ? float(1.0 / 3.0) : (i < 7 ? float(1.0 / 18.0) : float(1.0 / 36.0));
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(static_cast<F&&>(f));
+    }
+}
+
+// ------------------------------------------------------------- storage codecs
+enum Dtype { DT_F32 = 0, DT_F16 = 1, DT_BF16 = 2, DT_F64 = 3 };
+
+struct bf16_t { uint16_t bits; };
+
+// Storage policy: S is the element type in HBM, T the arithmetic type.  The 16-bit
+// forms store (f_i - w_i) * scale; scale is a power of two so the scaling is exact.
+template <class S> struct Codec;
+
+template <> struct Codec<float> {
+    using T = float;
+    static __device__ __forceinline__ float load(const float* p, size_t i, float, float) { return p[i]; }
+    static __device__ __forceinline__ void store(float* p, size_t i, float v, float, float) { p[i] = v; }
+};
+template <> struct Codec<double> {
+    using T = double;
+    static __device__ __forceinline__ double load(const double* p, size_t i, double, double) { return p[i]; }
+    static __device__ __forceinline__ void store(double* p, size_t i, double v, double, double) { p[i] = v; }
+};
+template <> struct Codec<_Float16> {
+    using T = float;
+    static __device__ __forceinline__ float load(const _Float16* p, size_t i, float w, float inv_scale) {
+        return w + (float)p[i] * inv_scale;
+    }
+    static __device__ __forceinline__ void store(_Float16* p, size_t i, float v, float w, float scale) {
+        p[i] = (_Float16)((v - w) * scale);
+    }
+};
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+template <> struct Codec<bf16_t> {
+    using T = float;
+    static __device__ __forceinline__ float load(const bf16_t* p, size_t i, float w, float inv_scale) {
+        return w + bf16_to_f32(p[i].bits) * inv_scale;
+    }
+    static __device__ __forceinline__ void store(bf16_t* p, size_t i, float v, float w, float scale) {
+        p[i].bits = f32_to_bf16((v - w) * scale);
+    }
+};
+
+// feq_i of lbm3d.py:33-36 with the operand order kept:
+//   (w*rho) * (((1 + cu) + (0.5*cu)*cu) - usq15),  cu = 3*((cx*ux + cy*uy) + cz*uz)
+// usq15 = 1.5*((ux*ux + uy*uy) + uz*uz) is shared by all i.  Terms with a zero
+// lattice coefficient are dropped: for finite operands x + (+-0) == x in every
+// case that reaches a store (the moment sums start at +0 and can never become -0),
+// so the result is bit-identical to the full expression.
+template <int I, class T>
+__device__ __forceinline__ T cdotu(T ux, T uy, T uz) {
+    constexpr int cx = CX[I], cy = CY[I], cz = CZ[I];
+    T s = T(0);
+    bool have = false;
+    if constexpr (cx != 0) { s = cx > 0 ? ux : -ux; have = true; }
+    if constexpr (cy != 0) {
+        T t = cy > 0 ? uy : -uy;
+        if (have) s = s + t; else s = t;
+        have = true;
+    }
+    if constexpr (cz != 0) {
+        T t = cz > 0 ? uz : -uz;
+        if (have) s = s + t; else s = t;
+    }
+    return s;
+}
+
+template <int I, class T>
+__device__ __forceinline__ T feq(T rho, T ux, T uy, T uz, T usq15) {
+    constexpr T w = wgt<T>(I);
+    if constexpr (I == 0) {
+        return (w * rho) * (T(1) - usq15);
+    } else {
+        const T cu = T(3) * cdotu<I>(ux, uy, uz);
+        return (w * rho) * (((T(1) + cu) + (T(0.5) * cu) * cu) - usq15);
+    }
+}
+
+// PCG hash (Jarzynski & Olano, JCGT 2020) for the optional init noise
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __forceinline__ float u01(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+
+// ------------------------------------------------------------- wave reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace sw

@@ -1,0 +1,323 @@
+// Single-lattice kernels and their C-ABI entry points (declared in include/stonewall_lbm.h).
+#include "sw_step.hpp"
+#include "sw_api_util.hpp"
+
+namespace sw {
+
+// ------------------------------------------------------------------ step
+template <class S, int YBC>
+__global__ __launch_bounds__(WG) void k_step(StepArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int z = blockIdx.x / a.nyb;
+    const int yb = blockIdx.x - z * a.nyb;
+    const int y = yb * ROWS + wave;
+    const int t = a.t + (a.t_dev ? *a.t_dev : 0);
+    double fx = 0.0, fz = 0.0;
+    if (y < a.ny) step_row<S, YBC>(a, y, z, t, lane, fx, fz);
+    block_force(a, fx, fz, z, yb, t);
+}
+
+// ------------------------------------------------------------------ link masks
+// flag is in the reference layout (nx, ny, nz), z contiguous (lbm3d.py:288-290).
+__global__ void k_prepare(const uint8_t* __restrict__ flag, uint32_t* __restrict__ links, int nx, int ny, int nz,
+                          int ybc, int* err) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    auto F = [&](int xx, int yy, int zz) { return flag[((size_t)xx * ny + yy) * nz + zz]; };
+    uint32_t m = 0;
+    if (F(x, y, z) != 0) {
+        m = SOLID_BIT;
+    } else if (z == 0) {
+        atomicOr(err, 1);                     // fluid in the bottom layer: its pull would leave the grid
+    } else if (x > 0) {                       // x == 0 is the inlet and does not pull
+        for (int i = 1; i < Q; i++) {
+            int xs = x - CX[i], ys = y - CY[i], zs = z - CZ[i];
+            if (ybc == Y_PERIODIC) ys = (ys + ny) % ny;
+            else if (ys < 0 || ys >= ny) ys = y;
+            if (xs >= nx) xs = x;
+            if (zs >= nz) zs = z;
+            const uint8_t fs = F(xs, ys, zs);
+            if (fs != 0) {
+                m |= 1u << i;
+                if (fs != 1) {
+                    if (GROUND_SLOT[i] >= 0) m |= 1u << (GROUND_BIT0 + GROUND_SLOT[i]);
+                    else atomicOr(err, 2);    // a non-wall solid reached by a link that is not cz=+1
+                }
+            }
+        }
+    }
+    links[((size_t)z * ny + y) * nx + x] = m;
+}
+
+// ------------------------------------------------------------------ init (lbm3d.py:39-54)
+template <class S>
+__global__ void k_init(S* __restrict__ f, const uint32_t* __restrict__ links, const float* __restrict__ uin,
+                       int nx, int ny, int nz, float noise, uint32_t seed, float scale) {
+    using C = Codec<S>;
+    using T = typename C::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    const uint32_t N = (uint32_t)nx * ny * nz;
+    const uint32_t self = ((uint32_t)z * ny + y) * nx + x;
+    T ux = T(0), uy = T(0), uz = T(0);
+    if (!(links[self] & SOLID_BIT)) {
+        const uint32_t key = ((uint32_t)x * 7919u + (uint32_t)y) * 7907u + (uint32_t)z;
+        const uint32_t h0 = pcg_hash(seed ^ pcg_hash(key));
+        const uint32_t h1 = pcg_hash(h0), h2 = pcg_hash(h1);
+        const T u0 = (T)uin[z], nz_ = (T)noise;
+        ux = u0 * (T(1) + nz_ * (T(2) * (T)u01(h0) - T(1)));
+        uy = (u0 * nz_) * (T(2) * (T)u01(h1) - T(1));
+        uz = (u0 * nz_) * (T(2) * (T)u01(h2) - T(1));
+    }
+    const T usq15 = T(1.5) * ((ux * ux + uy * uy) + uz * uz);
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        C::store(f, (size_t)i * N + self, feq<i, T>(T(1), ux, uy, uz, usq15), wgt<T>(i), (T)scale);
+    });
+}
+
+// ------------------------------------------------------------------ macro + accumulate (lbm3d.py:158-188)
+template <class S>
+__global__ void k_macro(const S* __restrict__ f, const uint32_t* __restrict__ links, int nx, int ny, int nz,
+                        float inv_scale, float* __restrict__ U, float* __restrict__ acc) {
+    using C = Codec<S>;
+    using T = typename C::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    const uint32_t N = (uint32_t)nx * ny * nz;
+    const uint32_t self = ((uint32_t)z * ny + y) * nx + x;
+    float u[4];
+    if (links[self] & SOLID_BIT) {
+        u[0] = 1.0f; u[1] = 0.0f; u[2] = 0.0f; u[3] = 0.0f;
+    } else {
+        T rho = T(0), mx = T(0), my = T(0), mz = T(0);
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            const T v = C::load(f, (size_t)i * N + self, wgt<T>(i), (T)inv_scale);
+            rho = rho + v;
+            if constexpr (CX[i] != 0) mx = CX[i] > 0 ? mx + v : mx - v;
+            if constexpr (CY[i] != 0) my = CY[i] > 0 ? my + v : my - v;
+            if constexpr (CZ[i] != 0) mz = CZ[i] > 0 ? mz + v : mz - v;
+        });
+        u[0] = (float)rho; u[1] = (float)(mx / rho); u[2] = (float)(my / rho); u[3] = (float)(mz / rho);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (U) U[(size_t)k * N + self] = u[k];
+        if (acc) acc[(size_t)k * N + self] = acc[(size_t)k * N + self] + u[k];
+    }
+}
+
+// ------------------------------------------------------------------ force rows
+// force[t][z][c] = sum over yb of partials[t % ring][z][yb][c], in a fixed order:
+// lane-strided sums, then a fixed butterfly; lane 0's value is written.
+__global__ void k_force_reduce(const double* __restrict__ partials, int ring, int t0, const int* t_dev, int nz,
+                               int nyb, double* __restrict__ force) {
+    const int s = blockIdx.y;                 // step within the batch
+    const int z = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int t = t0 + (t_dev ? *t_dev : 0) + s;
+    const double* p = partials + (((size_t)(t % ring) * nz + z) * nyb) * 2;
+    double sx = 0.0, sz = 0.0;
+    for (int k = lane; k < nyb; k += 64) { sx += p[2 * k]; sz += p[2 * k + 1]; }
+    sx = wave_sum(sx);
+    sz = wave_sum(sz);
+    if (lane == 0) {
+        force[((size_t)t * nz + z) * 2 + 0] = sx;
+        force[((size_t)t * nz + z) * 2 + 1] = sz;
+    }
+}
+
+// ------------------------------------------------------------------ layout transposes
+// [c][z][y][x]  <->  [c][x][y][z]; one (c, y) plane per blockIdx.z, 32x32 LDS tiles.
+template <class E, bool TO_REF>
+__global__ void k_transpose(const E* __restrict__ src, E* __restrict__ dst, int nx, int ny, int nz) {
+    __shared__ E tile[32][33];
+    const int cy = blockIdx.z;
+    const int c = cy / ny, y = cy - c * ny;
+    const size_t N = (size_t)nx * ny * nz;
+    const int tx = threadIdx.x, ty = threadIdx.y;       // 32 x 8
+    const int bx = blockIdx.x * 32, bz = blockIdx.y * 32;
+    // internal index (z, y, x) = (z*ny + y)*nx + x ; reference index (x, y, z) = (x*ny + y)*nz + z
+    if (TO_REF) {
+        for (int r = ty; r < 32; r += 8) {                  // read along x
+            const int x = bx + tx, z = bz + r;
+            if (x < nx && z < nz) tile[r][tx] = src[c * N + ((size_t)z * ny + y) * nx + x];
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {                  // write along z
+            const int z = bz + tx, x = bx + r;
+            if (x < nx && z < nz) dst[c * N + ((size_t)x * ny + y) * nz + z] = tile[tx][r];
+        }
+    } else {
+        for (int r = ty; r < 32; r += 8) {                  // read along z
+            const int z = bz + tx, x = bx + r;
+            if (x < nx && z < nz) tile[tx][r] = src[c * N + ((size_t)x * ny + y) * nz + z];
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {                  // write along x
+            const int x = bx + tx, z = bz + r;
+            if (x < nx && z < nz) dst[c * N + ((size_t)z * ny + y) * nx + x] = tile[r][tx];
+        }
+    }
+}
+
+}  // namespace sw
+
+using namespace sw;
+
+template <class S>
+static void launch_step(const StepArgs& a, int ybc, hipStream_t s) {
+    dim3 grid(a.nz * a.nyb), blk(WG);
+    if (ybc == Y_PERIODIC) hipLaunchKernelGGL((k_step<S, Y_PERIODIC>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_step<S, Y_FREESLIP>), grid, blk, 0, s, a);
+}
+
+
+// =================================================================== C ABI
+extern "C" {
+
+int sw_version(void) { return SW_VERSION; }
+
+const char* sw_last_error(void) { return last_error_message(); }
+
+int sw_rows_per_block(void) { return ROWS; }
+
+long long sw_partials_per_step(int ny, int nz) {
+    if (ny <= 0 || nz <= 0) return -1;
+    return (long long)nz * ((ny + ROWS - 1) / ROWS) * 2;
+}
+
+int sw_prepare_links(const uint8_t* flag, uint32_t* links, int nx, int ny, int nz, int ybc, void* stream) {
+    SW_CHECK_ARG(flag && links, "flag and links must be non-null");
+    SW_CHECK_RC(check_dims(nx, ny, nz));
+    SW_CHECK_ARG(ybc == Y_PERIODIC || ybc == Y_FREESLIP, "ybc must be 0 (periodic) or 1 (free-slip)");
+    hipStream_t s = (hipStream_t)stream;
+    int* err = nullptr;
+    SW_HIP(hipMalloc(&err, sizeof(int)));
+    int herr = 0;
+    hipError_t e = hipMemsetAsync(err, 0, sizeof(int), s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_prepare, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, s, flag, links, nx, ny, nz, ybc, err);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(err);
+    if (e != hipSuccess) return set_hip_error(e, "sw_prepare_links");
+    if (herr & 1) return set_error(SW_ERR_GEOMETRY, "sw_prepare_links: the bottom layer z=0 must be solid (ground)");
+    if (herr & 2)
+        return set_error(SW_ERR_GEOMETRY,
+                         "sw_prepare_links: a non-wall solid (flag>=2) is reached by a link other than cz=+1; "
+                         "only the ground layer may carry flag 2");
+    return SW_OK;
+}
+
+int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx, int ny, int nz, float noise,
+            unsigned int seed, float scale, void* stream) {
+    SW_CHECK_ARG(f && links && uin, "f, links and uin must be non-null");
+    SW_CHECK_RC(check_dims(nx, ny, nz));
+    SW_CHECK_RC(check_dtype(dtype, scale));
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((nx + 255) / 256, ny, nz), blk(256);
+    switch (dtype) {
+        case DT_F32: hipLaunchKernelGGL(k_init<float>, grid, blk, 0, s, (float*)f, links, uin, nx, ny, nz, noise, seed, 1.0f); break;
+        case DT_F64: hipLaunchKernelGGL(k_init<double>, grid, blk, 0, s, (double*)f, links, uin, nx, ny, nz, noise, seed, 1.0f); break;
+        case DT_F16: hipLaunchKernelGGL(k_init<_Float16>, grid, blk, 0, s, (_Float16*)f, links, uin, nx, ny, nz, noise, seed, scale); break;
+        case DT_BF16: hipLaunchKernelGGL(k_init<bf16_t>, grid, blk, 0, s, (bf16_t*)f, links, uin, nx, ny, nz, noise, seed, scale); break;
+    }
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
+            int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
+            int ybc, float scale, double* partials, int ring, void* stream) {
+    SW_CHECK_ARG(f_in && f_out && links && uin, "f_in, f_out, links and uin must be non-null");
+    SW_CHECK_ARG(f_in != f_out, "f_in and f_out must be different buffers (pull scheme)");
+    SW_CHECK_RC(check_dims(nx, ny, nz));
+    SW_CHECK_RC(check_dtype(dtype, scale));
+    SW_CHECK_ARG(ybc == Y_PERIODIC || ybc == Y_FREESLIP, "ybc must be 0 (periodic) or 1 (free-slip)");
+    SW_CHECK_ARG(!partials || ring > 0, "ring must be positive when partials are given");
+    SW_CHECK_ARG(t >= 0, "t must be non-negative");
+    StepArgs a{};
+    a.f_in = f_in; a.f_out = f_out; a.links = links; a.uin = uin; a.gust = gust;
+    a.partials = partials; a.t_dev = t_dev; a.t = t; a.ring = ring > 0 ? ring : 1;
+    a.nx = nx; a.ny = ny; a.nz = nz; a.nyb = (ny + ROWS - 1) / ROWS;
+    a.x_sponge = x_sponge; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
+    a.scale = scale; a.inv_scale = 1.0f / scale;
+    hipStream_t s = (hipStream_t)stream;
+    switch (dtype) {
+        case DT_F32: launch_step<float>(a, ybc, s); break;
+        case DT_F64: launch_step<double>(a, ybc, s); break;
+        case DT_F16: launch_step<_Float16>(a, ybc, s); break;
+        case DT_BF16: launch_step<bf16_t>(a, ybc, s); break;
+    }
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int ny, int nz,
+                    double* force, void* stream) {
+    SW_CHECK_ARG(partials && force, "partials and force must be non-null");
+    SW_CHECK_ARG(ring > 0 && nsteps > 0 && nsteps <= ring, "need 0 < nsteps <= ring");
+    SW_CHECK_ARG(ny > 0 && nz > 0 && t0 >= 0, "bad ny/nz/t0");
+    const int nyb = (ny + ROWS - 1) / ROWS;
+    hipLaunchKernelGGL(k_force_reduce, dim3(nz, nsteps), dim3(64), 0, (hipStream_t)stream, partials, ring, t0, t_dev,
+                       nz, nyb, force);
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale, float* U,
+             float* Uacc, void* stream) {
+    SW_CHECK_ARG(f && links && (U || Uacc), "f, links and one of U / Uacc must be non-null");
+    SW_CHECK_RC(check_dims(nx, ny, nz));
+    SW_CHECK_RC(check_dtype(dtype, scale));
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((nx + 255) / 256, ny, nz), blk(256);
+    const float inv = 1.0f / scale;
+    switch (dtype) {
+        case DT_F32: hipLaunchKernelGGL(k_macro<float>, grid, blk, 0, s, (const float*)f, links, nx, ny, nz, 1.0f, U, Uacc); break;
+        case DT_F64: hipLaunchKernelGGL(k_macro<double>, grid, blk, 0, s, (const double*)f, links, nx, ny, nz, 1.0f, U, Uacc); break;
+        case DT_F16: hipLaunchKernelGGL(k_macro<_Float16>, grid, blk, 0, s, (const _Float16*)f, links, nx, ny, nz, inv, U, Uacc); break;
+        case DT_BF16: hipLaunchKernelGGL(k_macro<bf16_t>, grid, blk, 0, s, (const bf16_t*)f, links, nx, ny, nz, inv, U, Uacc); break;
+    }
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+static int transpose(const void* src, void* dst, int elem_bytes, int ncomp, int nx, int ny, int nz, bool to_ref,
+                     void* stream) {
+    SW_CHECK_ARG(src && dst && src != dst, "src and dst must be distinct non-null buffers");
+    SW_CHECK_RC(check_dims(nx, ny, nz));
+    SW_CHECK_ARG(ncomp > 0 && (long long)ncomp * ny <= 65535LL, "ncomp * ny must be at most 65535");
+    dim3 grid((nx + 31) / 32, (nz + 31) / 32, ncomp * ny), blk(32, 8);
+    hipStream_t s = (hipStream_t)stream;
+#define SW_T(E)                                                                                         \
+    if (to_ref) hipLaunchKernelGGL((k_transpose<E, true>), grid, blk, 0, s, (const E*)src, (E*)dst, nx, ny, nz); \
+    else hipLaunchKernelGGL((k_transpose<E, false>), grid, blk, 0, s, (const E*)src, (E*)dst, nx, ny, nz);
+    switch (elem_bytes) {
+        case 1: SW_T(uint8_t) break;
+        case 2: SW_T(uint16_t) break;
+        case 4: SW_T(uint32_t) break;
+        case 8: SW_T(uint64_t) break;
+        default: return set_error(SW_ERR_INVALID, "elem_bytes must be 1, 2, 4 or 8");
+    }
+#undef SW_T
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_to_ref_layout(const void* src, void* dst, int elem_bytes, int ncomp, int nx, int ny, int nz, void* stream) {
+    return transpose(src, dst, elem_bytes, ncomp, nx, ny, nz, true, stream);
+}
+
+int sw_from_ref_layout(const void* src, void* dst, int elem_bytes, int ncomp, int nx, int ny, int nz, void* stream) {
+    return transpose(src, dst, elem_bytes, ncomp, nx, ny, nz, false, stream);
+}
+
+}  // extern "C"

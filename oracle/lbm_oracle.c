@@ -1,0 +1,480 @@
+/*
+ * CPU ORACLE -- test infrastructure only.
+ *
+ * A plain-C restatement of the D3Q19 regularized-BGK + Smagorinsky-LES kernels of
+ * stonewall3d/lbm3d.py.  It exists to CHECK the HIP kernels: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and the
+ * product path never routes through it.
+ *
+ * Layout and loop order follow the reference exactly: populations are stored as
+ * f[i][x][y][z] (Q, nx, ny, nz) with z contiguous (lbm3d.py:293), cells are visited
+ * x, y, z, and every floating-point expression keeps the reference's operand order
+ * (Python left-to-right precedence) so that, compiled with -ffp-contract=off, each
+ * rounding step is the one the reference's FP32 kernel performs.
+ *
+ * Functions and the reference lines they restate:
+ *   lattice tables  C, W, OPP, MIRZ, flags     lbm3d.py:20-30
+ *   feq()                                      lbm3d.py:33-36   (feq_i)
+ *   oracle_init_*                              lbm3d.py:39-54   (k_init; see NOISE below)
+ *   oracle_step_*                              lbm3d.py:57-155  (k_step)
+ *   oracle_macro_*                             lbm3d.py:158-188 (k_macro + k_accum)
+ *
+ * Additions that have no counterpart in the reference (new boundary / inflow
+ * options and storage forms; each is a no-op in "reference mode"):
+ *   - gust:     the inlet speed is uin[z]*gust (gust = 1 reproduces lbm3d.py:68);
+ *   - ybc = 1:  free-slip side walls (y-mirror, by analogy with the z-mirror
+ *               at lbm3d.py:84-86); ybc = 0 is the reference's periodic y (:79);
+ *   - storage:  populations stored as FP16 / BF16 shifted by their weight,
+ *               value = w_i + decode(h) / scale, with all arithmetic in FP32;
+ *   - FP64:     the same algorithm with double everywhere;
+ *   - force:    each cell's FP32 (fxw, fzw) exactly as lbm3d.py:89-101, summed per
+ *               z level in FP64 in cell order instead of by FP32 atomics.
+ * NOISE: the reference draws init noise from Warp's RNG (lbm3d.py:48-52), which
+ * cannot be reproduced without Warp; here the noise comes from the PCG hash of
+ * Jarzynski & Olano (JCGT 2020) keyed by the reference's cell key
+ * (x*7919+y)*7907+z.  The noise stream is therefore parity-unpinned against the
+ * reference; the HIP init kernel uses the same hash so GPU and oracle agree.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define Q 19
+
+static const int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+static const int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+/* OPP / MIRZ as built at lbm3d.py:26-27; MIRY is the y analogue of MIRZ. */
+static const int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+static const int MIRZ[Q] = {0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16};
+static const int MIRY[Q] = {0, 1, 2, 4, 3, 5, 6, 9, 10, 7, 8, 11, 12, 13, 14, 18, 17, 16, 15};
+
+/* W as float32, exactly as np.array([1/3]+[1/18]*6+[1/36]*12, dtype=float32). */
+static float W32(int i) { return i == 0 ? (float)(1.0 / 3.0) : (i < 7 ? (float)(1.0 / 18.0) : (float)(1.0 / 36.0)); }
+static double W64(int i) { return i == 0 ? 1.0 / 3.0 : (i < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
+
+int oracle_tables(int* cx, int* cy, int* cz, int* opp, int* mirz, int* miry, float* w) {
+    for (int i = 0; i < Q; i++) {
+        cx[i] = CX[i]; cy[i] = CY[i]; cz[i] = CZ[i];
+        opp[i] = OPP[i]; mirz[i] = MIRZ[i]; miry[i] = MIRY[i]; w[i] = W32(i);
+    }
+    return Q;
+}
+
+/* ---------------------------------------------------------------- FP16 / BF16 codecs
+ * Round-to-nearest-even conversions written from the IEEE-754 binary16 and the
+ * bfloat16 (upper 16 bits of binary32) definitions.                              */
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+uint16_t oracle_f32_to_f16(float f) {
+    uint32_t u = f2u(f);
+    uint32_t sign = (u >> 16) & 0x8000u;
+    uint32_t ex = (u >> 23) & 0xFFu;
+    uint32_t man = u & 0x7FFFFFu;
+    if (ex == 0xFFu) return (uint16_t)(sign | 0x7C00u | (man ? 0x200u : 0u));
+    int e = (int)ex - 127 + 15;
+    if (e >= 31) return (uint16_t)(sign | 0x7C00u);              /* overflow -> inf */
+    if (e <= 0) {                                                 /* subnormal / zero */
+        if (e < -10) return (uint16_t)sign;
+        man |= 0x800000u;
+        int shift = 14 - e;                                       /* 14..24 */
+        uint32_t half = man >> shift;
+        uint32_t rem = man & ((1u << shift) - 1u);
+        uint32_t mid = 1u << (shift - 1);
+        if (rem > mid || (rem == mid && (half & 1u))) half++;
+        return (uint16_t)(sign | half);
+    }
+    uint32_t h = ((uint32_t)e << 10) | (man >> 13);
+    uint32_t rem = man & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;       /* may carry into exp: ok */
+    return (uint16_t)(sign | h);
+}
+
+float oracle_f16_to_f32(uint16_t h) {
+    uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    uint32_t ex = (h >> 10) & 0x1Fu;
+    uint32_t man = h & 0x3FFu;
+    if (ex == 0x1Fu) return u2f(sign | 0x7F800000u | (man << 13));
+    if (ex == 0) {
+        if (man == 0) return u2f(sign);
+        float v = (float)man * (1.0f / 16777216.0f);              /* man * 2^-24, exact */
+        return sign ? -v : v;
+    }
+    return u2f(sign | ((ex - 15u + 127u) << 23) | (man << 13));
+}
+
+uint16_t oracle_f32_to_bf16(float f) {
+    uint32_t u = f2u(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+float oracle_bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
+
+/* ---------------------------------------------------------------- PCG hash noise */
+static uint32_t pcg_hash(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+static float u01(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+
+/* ---------------------------------------------------------------- storage codecs */
+enum { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2, ST_F64 = 3 };
+
+/* ================================================================ FP32 / shifted storage
+ * One body for the three FP32-arithmetic storage forms. */
+static inline float load32(const void* f, size_t idx, int st, float w, float inv_scale) {
+    if (st == ST_F32) return ((const float*)f)[idx];
+    float d = (st == ST_F16) ? oracle_f16_to_f32(((const uint16_t*)f)[idx])
+                             : oracle_bf16_to_f32(((const uint16_t*)f)[idx]);
+    return w + d * inv_scale;
+}
+static inline void store32(void* f, size_t idx, int st, float v, float w, float scale) {
+    if (st == ST_F32) { ((float*)f)[idx] = v; return; }
+    float d = (v - w) * scale;
+    ((uint16_t*)f)[idx] = (st == ST_F16) ? oracle_f32_to_f16(d) : oracle_f32_to_bf16(d);
+}
+
+/* feq_i, lbm3d.py:33-36, float32 */
+static inline float feq32(float w, float cx, float cy, float cz, float rho, float ux, float uy, float uz) {
+    float cu = 3.0f * (cx * ux + cy * uy + cz * uz);
+    return w * rho * (1.0f + cu + 0.5f * cu * cu - 1.5f * (ux * ux + uy * uy + uz * uz));
+}
+static inline double feq64(double w, double cx, double cy, double cz, double rho, double ux, double uy, double uz) {
+    double cu = 3.0 * (cx * ux + cy * uy + cz * uz);
+    return w * rho * (1.0 + cu + 0.5 * cu * cu - 1.5 * (ux * ux + uy * uy + uz * uz));
+}
+
+#define IDX(i, x, y, z) ((((size_t)(i) * nx + (size_t)(x)) * ny + (size_t)(y)) * nz + (size_t)(z))
+#define FIDX(x, y, z) ((((size_t)(x)) * ny + (size_t)(y)) * nz + (size_t)(z))
+
+/* k_init (lbm3d.py:39-54) with the PCG noise documented above. */
+int oracle_init_f32(void* f, int st, const uint8_t* flag, const float* uin, int nx, int ny, int nz,
+                    float noise, uint32_t seed, float scale) {
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+                if (flag[FIDX(x, y, z)] == 0) {
+                    uint32_t key = ((uint32_t)x * 7919u + (uint32_t)y) * 7907u + (uint32_t)z;
+                    uint32_t h0 = pcg_hash(seed ^ pcg_hash(key));
+                    uint32_t h1 = pcg_hash(h0), h2 = pcg_hash(h1);
+                    float u0 = uin[z];
+                    ux = u0 * (1.0f + noise * (2.0f * u01(h0) - 1.0f));
+                    uy = u0 * noise * (2.0f * u01(h1) - 1.0f);
+                    uz = u0 * noise * (2.0f * u01(h2) - 1.0f);
+                }
+                for (int i = 0; i < Q; i++)
+                    store32(f, IDX(i, x, y, z), st,
+                            feq32(W32(i), (float)CX[i], (float)CY[i], (float)CZ[i], 1.0f, ux, uy, uz), W32(i), scale);
+            }
+    return 0;
+}
+
+/* Source of link i for cell (x,y,z): lbm3d.py:78-86, plus the optional y-mirror. */
+static inline void source(int i, int x, int y, int z, int nx, int ny, int nz, int ybc,
+                          int* xs, int* ys, int* zs, int* j) {
+    *xs = x - CX[i];
+    *zs = z - CZ[i];
+    *j = i;
+    if (ybc == 0) {
+        *ys = (y - CY[i] + ny) % ny;
+    } else {
+        *ys = y - CY[i];
+        if (*ys < 0 || *ys >= ny) { *ys = y; *j = MIRY[*j]; }
+    }
+    if (*xs >= nx) *xs = x;                                /* outlet: zero-gradient */
+    if (*zs >= nz) { *zs = z; *j = MIRZ[*j]; }             /* top: free-slip */
+}
+
+/* k_step (lbm3d.py:57-155), FP32 arithmetic over any of the FP32 storage forms.
+ * force_z (nz*2 doubles, may be NULL) receives sum over cells of the per-cell FP32
+ * (fxw, fzw), summed in FP64 in x, y order per z (deterministic for any thread count). */
+int oracle_step_f32(const void* f_in, void* f_out, int st, const uint8_t* flag, const float* uin, float gust,
+                    int nx, int ny, int nz, float nu0, float cs, int x_sponge, float nu_sponge, int ybc,
+                    float scale, double* force_z) {
+    const float inv_scale = 1.0f / scale;
+    double* part = NULL;
+    if (force_z) {
+        part = (double*)calloc((size_t)nx * nz * 2, sizeof(double));
+        if (!part) return -1;
+    }
+    const float sq2 = sqrtf(2.0f);
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                if (flag[FIDX(x, y, z)] != 0) continue;                      /* :65-66 */
+                if (x == 0) {                                                 /* :67-71 */
+                    float u0 = uin[z] * gust;                                 /* gust = 1 is exact */
+                    for (int i = 0; i < Q; i++)
+                        store32(f_out, IDX(i, x, y, z), st,
+                                feq32(W32(i), (float)CX[i], (float)CY[i], (float)CZ[i], 1.0f, u0, 0.0f, 0.0f),
+                                W32(i), scale);
+                    continue;
+                }
+                float fl[Q];
+                float fxw = 0.0f, fzw = 0.0f;
+                int hit = 0;
+                for (int i = 0; i < Q; i++) {                                 /* :77-98 */
+                    int xs, ys, zs, j;
+                    source(i, x, y, z, nx, ny, nz, ybc, &xs, &ys, &zs, &j);
+                    float val = 0.0f;
+                    uint8_t fs = flag[FIDX(xs, ys, zs)];
+                    if (fs != 0) {
+                        val = load32(f_in, IDX(OPP[i], x, y, z), st, W32(OPP[i]), inv_scale);
+                        if (fs == 1) {
+                            float fr = val - W32(i);
+                            fxw = fxw - 2.0f * fr * (float)CX[i];
+                            fzw = fzw - 2.0f * fr * (float)CZ[i];
+                            hit = 1;
+                        }
+                    } else {
+                        val = load32(f_in, IDX(j, xs, ys, zs), st, W32(j), inv_scale);
+                    }
+                    fl[i] = val;
+                }
+                if (hit && part) {
+                    part[((size_t)x * nz + z) * 2 + 0] += (double)fxw;
+                    part[((size_t)x * nz + z) * 2 + 1] += (double)fzw;
+                }
+                float rho = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;        /* :103-127 */
+                float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
+                for (int i = 0; i < Q; i++) {
+                    float v = fl[i];
+                    float ex = (float)CX[i], ey = (float)CY[i], ez = (float)CZ[i];
+                    rho += v;
+                    mx += v * ex;
+                    my += v * ey;
+                    mz += v * ez;
+                    pxx += v * ex * ex;
+                    pyy += v * ey * ey;
+                    pzz += v * ez * ez;
+                    pxy += v * ex * ey;
+                    pxz += v * ex * ez;
+                    pyz += v * ey * ez;
+                }
+                float ux = mx / rho, uy = my / rho, uz = mz / rho;           /* :128-138 */
+                float third = rho / 3.0f;
+                float nxx = pxx - third - rho * ux * ux;
+                float nyy = pyy - third - rho * uy * uy;
+                float nzz = pzz - third - rho * uz * uz;
+                float nxy = pxy - rho * ux * uy;
+                float nxz = pxz - rho * ux * uz;
+                float nyz = pyz - rho * uy * uz;
+                float nu = nu0;                                                /* :140-147 */
+                if (x > x_sponge) {
+                    float s = (float)(x - x_sponge) / (float)(nx - x_sponge);
+                    nu = nu0 + nu_sponge * s * s;
+                }
+                float tau0 = 3.0f * nu + 0.5f;
+                float qn = sqrtf(nxx * nxx + nyy * nyy + nzz * nzz + 2.0f * (nxy * nxy + nxz * nxz + nyz * nyz));
+                float tau = 0.5f * (tau0 + sqrtf(tau0 * tau0 + 18.0f * sq2 * cs * cs * qn / rho));
+                float keep = 1.0f - 1.0f / tau;
+                for (int i = 0; i < Q; i++) {                                 /* :149-155 */
+                    float ex = (float)CX[i], ey = (float)CY[i], ez = (float)CZ[i];
+                    float qq = ((ex * ex - 1.0f / 3.0f) * nxx + (ey * ey - 1.0f / 3.0f) * nyy
+                                + (ez * ez - 1.0f / 3.0f) * nzz
+                                + 2.0f * (ex * ey * nxy + ex * ez * nxz + ey * ez * nyz));
+                    store32(f_out, IDX(i, x, y, z), st,
+                            feq32(W32(i), ex, ey, ez, rho, ux, uy, uz) + keep * W32(i) * 4.5f * qq, W32(i), scale);
+                }
+            }
+    if (force_z) {
+        for (int z = 0; z < nz; z++) { force_z[2 * z] = 0.0; force_z[2 * z + 1] = 0.0; }
+        for (int x = 0; x < nx; x++)
+            for (int z = 0; z < nz; z++) {
+                force_z[2 * z] += part[((size_t)x * nz + z) * 2 + 0];
+                force_z[2 * z + 1] += part[((size_t)x * nz + z) * 2 + 1];
+            }
+        free(part);
+    }
+    return 0;
+}
+
+/* k_macro + k_accum (lbm3d.py:158-188), FP32.  U and acc are (4, nx, ny, nz). */
+int oracle_macro_f32(const void* f, int st, const uint8_t* flag, int nx, int ny, int nz, float scale,
+                     float* U, float* acc) {
+    const float inv_scale = 1.0f / scale;
+    const size_t N = (size_t)nx * ny * nz;
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                size_t c = FIDX(x, y, z);
+                float u[4];
+                if (flag[c] != 0) {
+                    u[0] = 1.0f; u[1] = 0.0f; u[2] = 0.0f; u[3] = 0.0f;
+                } else {
+                    float rho = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+                    for (int i = 0; i < Q; i++) {
+                        float v = load32(f, IDX(i, x, y, z), st, W32(i), inv_scale);
+                        rho += v;
+                        mx += v * (float)CX[i];
+                        my += v * (float)CY[i];
+                        mz += v * (float)CZ[i];
+                    }
+                    u[0] = rho; u[1] = mx / rho; u[2] = my / rho; u[3] = mz / rho;
+                }
+                for (int k = 0; k < 4; k++) {
+                    if (U) U[k * N + c] = u[k];
+                    if (acc) acc[k * N + c] = acc[k * N + c] + u[k];
+                }
+            }
+    return 0;
+}
+
+/* ================================================================ FP64 (the same algorithm in double) */
+int oracle_init_f64(double* f, const uint8_t* flag, const float* uin, int nx, int ny, int nz,
+                    float noise_f, uint32_t seed) {
+    const double noise = (double)noise_f;
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                double ux = 0.0, uy = 0.0, uz = 0.0;
+                if (flag[FIDX(x, y, z)] == 0) {
+                    uint32_t key = ((uint32_t)x * 7919u + (uint32_t)y) * 7907u + (uint32_t)z;
+                    uint32_t h0 = pcg_hash(seed ^ pcg_hash(key));
+                    uint32_t h1 = pcg_hash(h0), h2 = pcg_hash(h1);
+                    double u0 = (double)uin[z];
+                    ux = u0 * (1.0 + noise * (2.0 * (double)u01(h0) - 1.0));
+                    uy = u0 * noise * (2.0 * (double)u01(h1) - 1.0);
+                    uz = u0 * noise * (2.0 * (double)u01(h2) - 1.0);
+                }
+                for (int i = 0; i < Q; i++)
+                    f[IDX(i, x, y, z)] = feq64(W64(i), CX[i], CY[i], CZ[i], 1.0, ux, uy, uz);
+            }
+    return 0;
+}
+
+int oracle_step_f64(const double* f_in, double* f_out, const uint8_t* flag, const float* uin, double gust,
+                    int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge, int ybc,
+                    double* force_z) {
+    double* part = NULL;
+    if (force_z) {
+        part = (double*)calloc((size_t)nx * nz * 2, sizeof(double));
+        if (!part) return -1;
+    }
+    const double sq2 = sqrt(2.0);
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                if (flag[FIDX(x, y, z)] != 0) continue;
+                if (x == 0) {
+                    double u0 = (double)uin[z] * gust;
+                    for (int i = 0; i < Q; i++)
+                        f_out[IDX(i, x, y, z)] = feq64(W64(i), CX[i], CY[i], CZ[i], 1.0, u0, 0.0, 0.0);
+                    continue;
+                }
+                double fl[Q];
+                double fxw = 0.0, fzw = 0.0;
+                int hit = 0;
+                for (int i = 0; i < Q; i++) {
+                    int xs, ys, zs, j;
+                    source(i, x, y, z, nx, ny, nz, ybc, &xs, &ys, &zs, &j);
+                    double val;
+                    uint8_t fs = flag[FIDX(xs, ys, zs)];
+                    if (fs != 0) {
+                        val = f_in[IDX(OPP[i], x, y, z)];
+                        if (fs == 1) {
+                            double fr = val - W64(i);
+                            fxw = fxw - 2.0 * fr * CX[i];
+                            fzw = fzw - 2.0 * fr * CZ[i];
+                            hit = 1;
+                        }
+                    } else {
+                        val = f_in[IDX(j, xs, ys, zs)];
+                    }
+                    fl[i] = val;
+                }
+                if (hit && part) {
+                    part[((size_t)x * nz + z) * 2 + 0] += fxw;
+                    part[((size_t)x * nz + z) * 2 + 1] += fzw;
+                }
+                double rho = 0, mx = 0, my = 0, mz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+                for (int i = 0; i < Q; i++) {
+                    double v = fl[i], ex = CX[i], ey = CY[i], ez = CZ[i];
+                    rho += v; mx += v * ex; my += v * ey; mz += v * ez;
+                    pxx += v * ex * ex; pyy += v * ey * ey; pzz += v * ez * ez;
+                    pxy += v * ex * ey; pxz += v * ex * ez; pyz += v * ey * ez;
+                }
+                double ux = mx / rho, uy = my / rho, uz = mz / rho;
+                double third = rho / 3.0;
+                double nxx = pxx - third - rho * ux * ux;
+                double nyy = pyy - third - rho * uy * uy;
+                double nzz = pzz - third - rho * uz * uz;
+                double nxy = pxy - rho * ux * uy;
+                double nxz = pxz - rho * ux * uz;
+                double nyz = pyz - rho * uy * uz;
+                double nu = nu0;
+                if (x > x_sponge) {
+                    double s = (double)(x - x_sponge) / (double)(nx - x_sponge);
+                    nu = nu0 + nu_sponge * s * s;
+                }
+                double tau0 = 3.0 * nu + 0.5;
+                double qn = sqrt(nxx * nxx + nyy * nyy + nzz * nzz + 2.0 * (nxy * nxy + nxz * nxz + nyz * nyz));
+                double tau = 0.5 * (tau0 + sqrt(tau0 * tau0 + 18.0 * sq2 * cs * cs * qn / rho));
+                double keep = 1.0 - 1.0 / tau;
+                for (int i = 0; i < Q; i++) {
+                    double ex = CX[i], ey = CY[i], ez = CZ[i];
+                    double qq = ((ex * ex - 1.0 / 3.0) * nxx + (ey * ey - 1.0 / 3.0) * nyy
+                                 + (ez * ez - 1.0 / 3.0) * nzz
+                                 + 2.0 * (ex * ey * nxy + ex * ez * nxz + ey * ez * nyz));
+                    f_out[IDX(i, x, y, z)] = feq64(W64(i), ex, ey, ez, rho, ux, uy, uz) + keep * W64(i) * 4.5 * qq;
+                }
+            }
+    if (force_z) {
+        for (int z = 0; z < nz; z++) { force_z[2 * z] = 0.0; force_z[2 * z + 1] = 0.0; }
+        for (int x = 0; x < nx; x++)
+            for (int z = 0; z < nz; z++) {
+                force_z[2 * z] += part[((size_t)x * nz + z) * 2 + 0];
+                force_z[2 * z + 1] += part[((size_t)x * nz + z) * 2 + 1];
+            }
+        free(part);
+    }
+    return 0;
+}
+
+int oracle_macro_f64(const double* f, const uint8_t* flag, int nx, int ny, int nz, double* U) {
+    const size_t N = (size_t)nx * ny * nz;
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                size_t c = FIDX(x, y, z);
+                if (flag[c] != 0) { U[c] = 1.0; U[N + c] = 0.0; U[2 * N + c] = 0.0; U[3 * N + c] = 0.0; continue; }
+                double rho = 0, mx = 0, my = 0, mz = 0;
+                for (int i = 0; i < Q; i++) {
+                    double v = f[IDX(i, x, y, z)];
+                    rho += v; mx += v * CX[i]; my += v * CY[i]; mz += v * CZ[i];
+                }
+                U[c] = rho; U[N + c] = mx / rho; U[2 * N + c] = my / rho; U[3 * N + c] = mz / rho;
+            }
+    return 0;
+}
+
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

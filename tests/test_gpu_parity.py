@@ -1,0 +1,181 @@
+"""HIP kernels vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Bar: FP32 / FP16 / BF16 storage bit-identical populations to the oracle (both compiled
+without FMA contraction, same operand order); force rows (FP64 sums in different fixed
+orders) within 1e-12 relative; FP64 within 1e-14.  The workload-level bar (populations
+within 1e-5 relative after 1000 steps, mean Cd within 0.5 %) is asserted on the 160x80x40
+box-wall lattice in both y-boundary modes.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import c_oracle as co
+from stonewall3d import geometry, workloads
+from stonewall3d.inflow import baseline_profile, gust_table, make_gust
+from stonewall3d.lattice import flag_from_mask
+from stonewall3d.lbm3d import Simulation
+
+pytestmark = pytest.mark.gpu
+
+SCALE = {"f32": 1.0, "f16": 1024.0, "bf16": 1.0, "f64": 1.0}
+
+
+def small_case(nx=48, ny=13, nz=18, H=8, seed=3):
+    wall, _ = geometry.rounded_box_wall(nx, ny, nz, 12, 5, H, 0.3, seed, size=(0.4, 0.8))
+    wall[30, 4, 3] = True                                # a lone stone
+    flag = flag_from_mask(wall)
+    uin = baseline_profile(nz, H)
+    return flag, uin
+
+
+def fluid_mask(flag):
+    return (flag == 0)[None]
+
+
+def run_both(flag, uin, steps, storage, ybc, gust=None, noise=0.02, seed=11, nu0=2e-4, x_sponge=None):
+    nx = flag.shape[0]
+    xs = nx - 10 if x_sponge is None else x_sponge
+    sc = SCALE[storage]
+    sim = Simulation(flag, uin, nu0=nu0, cs=0.16, x_sponge=xs, ybc=ybc, dtype=storage, scale=sc, gust=gust,
+                     n_steps=steps, ring=7)
+    sim.init(noise=noise, seed=seed)
+    f = co.init(flag, uin, storage, noise=noise, seed=seed, scale=sc)
+    g0 = sim.populations()
+    assert np.array_equal(np.where(fluid_mask(flag), g0, 0), np.where(fluid_mask(flag), f, 0)), "init differs"
+    forces = []
+    for t in range(steps):
+        g = 1.0 if gust is None else float(gust[t])
+        f, F = co.step(f, flag, uin, nu0, 0.16, xs, 0.05, gust=g, ybc=ybc, storage=storage, scale=sc)
+        forces.append(F)
+    sim.step(steps)
+    return sim, f, np.array(forces)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("ybc", [0, 1])
+def test_step_bit_identical(gpu, storage, ybc):
+    flag, uin = small_case()
+    gust = gust_table(make_gust(5), 40)
+    sim, f, F = run_both(flag, uin, 40, storage, ybc, gust=gust)
+    g = sim.populations()
+    fl = fluid_mask(flag)
+    diff = np.where(fl, g != f, False)
+    assert not diff.any(), f"{diff.sum()} populations differ ({storage}, ybc={ybc})"
+    Fg = sim.force_rows()
+    assert Fg.shape == F.shape
+    assert np.abs(F).max() > 0
+    assert np.allclose(Fg, F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
+
+
+def test_fp64_matches_oracle(gpu):
+    flag, uin = small_case()
+    sim, f, F = run_both(flag, uin, 25, "f64", 1, gust=gust_table(make_gust(2), 25))
+    g = sim.populations()
+    fl = fluid_mask(flag)
+    rel = np.abs(np.where(fl, g - f, 0)).max() / np.abs(f).max()
+    print("fp64 max rel diff", rel, "bitwise", not np.where(fl, g != f, False).any())
+    assert rel < 1e-14
+    assert np.allclose(sim.force_rows(), F, rtol=1e-12, atol=1e-16)
+
+
+def test_macro_and_accumulate_bit_identical(gpu):
+    flag, uin = small_case()
+    sim, f, _ = run_both(flag, uin, 5, "f32", 0)
+    U = sim.macro()
+    Uo = co.macro(f, flag)
+    assert np.array_equal(U, Uo)
+    sim.accumulate()
+    sim.accumulate()
+    acc = np.zeros_like(Uo)
+    co.macro(f, flag, acc=acc)
+    co.macro(f, flag, acc=acc)
+    assert np.array_equal(sim.mean_field() * 2, acc)
+
+
+def test_layout_round_trip(gpu):
+    flag, uin = small_case()
+    sim = Simulation(flag, uin, nu0=2e-4, n_steps=4)
+    sim.init(noise=0.02, seed=1)
+    sim.step(3)
+    p = sim.populations()
+    sim2 = Simulation(flag, uin, nu0=2e-4, n_steps=4)
+    sim2.load_populations(p)
+    assert np.array_equal(sim2.populations(), p)
+
+
+def test_prepare_rejects_fluid_ground(gpu):
+    from stonewall3d._native import SwError
+    flag = np.zeros((8, 4, 6), np.uint8)
+    with pytest.raises(SwError) as e:
+        Simulation(flag, np.zeros(6, np.float32), nu0=1e-3)
+    assert e.value.code == 1002
+
+
+def test_deterministic_and_ring_independent(gpu):
+    flag, uin = small_case()
+    out = []
+    for ring in (1, 3, 64):
+        sim = Simulation(flag, uin, nu0=2e-4, x_sponge=38, ybc="freeslip", n_steps=30, ring=ring,
+                         gust=gust_table(make_gust(1), 30))
+        sim.init(noise=0.02, seed=4)
+        sim.step(30)
+        out.append((sim.populations(), sim.force_rows()))
+    for p, F in out[1:]:
+        assert np.array_equal(p, out[0][0])
+        assert np.array_equal(F, out[0][1])
+
+
+@pytest.mark.parametrize("ybc", ["freeslip", "periodic"])
+def test_box_wall_1000_steps(gpu, ybc):
+    """The 160x80x40 box-wall workload, 1000 steps: populations within 1e-5 relative of
+    the oracle (bit-identical expected) and mean Cd within 0.5 %."""
+    w = workloads.box_c0(n_steps=1000, ybc=ybc)
+    flag, uin = w.flag, w.uin
+    sim = Simulation(flag, uin, **w.sim_kwargs())
+    sim.init()
+    f = co.init(flag, uin, "f32")
+    F = []
+    yb = 1 if ybc == "freeslip" else 0
+    for t in range(1000):
+        f, Ft = co.step(f, flag, uin, w.nu0, w.cs, w.x_sponge, w.nu_sponge, ybc=yb)
+        F.append(Ft)
+    sim.step(1000)
+    g = sim.populations()
+    fl = fluid_mask(flag)
+    rel = np.abs(np.where(fl, g - f, 0)) / np.abs(f).max()
+    n_diff = int(np.where(fl, g != f, False).sum())
+    print(f"box wall {ybc}: max rel diff {rel.max():.3e}, differing populations {n_diff}")
+    assert rel.max() <= 1e-5
+    cd_o = np.array(F)[:, :, 0].sum(1) / (0.5 * w.UH ** 2 * w.H * flag.shape[1])
+    cd_g = sim.coefficients(w.UH, w.H)["Cd"]
+    assert abs(cd_g[100:].mean() - cd_o[100:].mean()) <= 0.005 * abs(cd_o[100:].mean())
+    assert n_diff == 0
+
+
+def test_headline_lattice_first_steps_bit_identical(gpu):
+    """640x320x160 porous wall (20 % void, gusting inlet): 3 steps bit-identical to the
+    oracle at full size, and two GPU runs identical."""
+    w = workloads.porous_wall(H=40, porosity=0.2, seed=0, n_steps=3)
+    co.set_threads(16)
+    sims = []
+    for _ in range(2):
+        sim = Simulation(w.flag, w.uin, **w.sim_kwargs())
+        sim.init(noise=0.02, seed=0)
+        sim.step(3)
+        sims.append(sim)
+    f = co.init(w.flag, w.uin, "f32", noise=0.02, seed=0)
+    yb = 1
+    F = []
+    for t in range(3):
+        f, Ft = co.step(f, w.flag, w.uin, w.nu0, w.cs, w.x_sponge, w.nu_sponge, gust=float(w.gust[t]), ybc=yb)
+        F.append(Ft)
+    g = sims[0].populations()
+    fl = fluid_mask(w.flag)
+    assert not np.where(fl, g != f, False).any()
+    assert np.array_equal(g, sims[1].populations())
+    Fg = sims[0].force_rows()
+    assert np.allclose(Fg, np.array(F), rtol=1e-11, atol=1e-14)
+    assert np.array_equal(Fg, sims[1].force_rows())
+    del sims
+    torch.cuda.empty_cache()

@@ -1,0 +1,78 @@
+"""The C-ABI library loads, exports every symbol include/*.h declares, and rejects bad
+arguments with library error codes before touching a device (runs without a GPU)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from stonewall3d import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(sw_\w+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 11
+    L = ctypes.CDLL(nat.LIB_PATH)
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in nat.SIGNATURES, f"{n} has no ctypes signature"
+    assert set(nat.SIGNATURES) <= set(names)
+
+
+def test_version_and_sizes():
+    L = nat.lib()
+    assert L.sw_version() == 100
+    assert L.sw_rows_per_block() == 4
+    assert L.sw_partials_per_step(320, 160) == 160 * 80 * 2
+    assert L.sw_partials_per_step(321, 3) == 3 * 81 * 2
+    assert L.sw_partials_per_step(0, 3) == -1
+
+
+def test_argument_errors_without_device():
+    L = nat.lib()
+    # null buffers
+    rc = L.sw_step(None, None, 0, None, None, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0, 1.0, None, 1, None)
+    assert rc == 1001
+    assert b"non-null" in L.sw_last_error()
+    fake = ctypes.c_void_p(0x1000)
+    # bad dims / dtype / scale / ybc / too large
+    args = dict(nx=8, ny=8, nz=8, dtype=0, scale=1.0, ybc=0)
+
+    def step(**kw):
+        a = dict(args, **kw)
+        return L.sw_step(fake, ctypes.c_void_p(0x2000), a["dtype"], fake, fake, None, 0, None, a["nx"], a["ny"],
+                         a["nz"], 1e-3, 0.16, 8, 0.05, a["ybc"], a["scale"], None, 1, None)
+
+    assert step(nx=1) == 1001
+    assert step(dtype=7) == 1001
+    assert step(dtype=1, scale=3.0) == 1001          # 16-bit scale must be a power of two
+    assert step(ybc=5) == 1001
+    assert step(nx=2000, ny=2000, nz=100) == 1003
+    assert b"113,025,455" in L.sw_last_error()
+    # in-place is rejected (pull scheme needs two buffers)
+    assert L.sw_step(fake, fake, 0, fake, fake, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0, 1.0, None, 1,
+                     None) == 1001
+    # partials without a ring
+    assert L.sw_step(fake, ctypes.c_void_p(0x2000), 0, fake, fake, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0,
+                     1.0, fake, 0, None) == 1001
+    assert L.sw_force_reduce(fake, 4, 0, None, 5, 8, 8, fake, None) == 1001
+    assert L.sw_to_ref_layout(fake, fake, 4, 19, 8, 8, 8, None) == 1001
+    assert L.sw_to_ref_layout(fake, ctypes.c_void_p(0x2000), 3, 19, 8, 8, 8, None) == 1001
+
+
+def test_check_raises_with_message():
+    with pytest.raises(nat.SwError) as e:
+        nat.call("sw_macro", None, 0, None, 8, 8, 8, 1.0, None, None, None)
+    assert e.value.code == 1001
