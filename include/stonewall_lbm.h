@@ -89,6 +89,13 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
             int x_sponge, double nu_sponge, int ybc, float scale, double* partials, int ring,
             void* stream);
 
+/* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
+ * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept
+ * per dtype for the process: 0 = one 64-cell chunk per wavefront iteration, 1 = two. */
+int sw_step_variants(void);
+int sw_set_step_variant(int dtype, int variant);
+int sw_get_step_variant(int dtype);
+
 /* force[t][z][0..1] (FP64 Fx, Fz per height level) for t = t0 + *t_dev .. + nsteps - 1,
  * summed in a fixed order from the partials ring.  Replaces the FP32 atomic scatter
  * wp.atomic_add(force, t, z, c, ...) at lbm3d.py:99-101 (force array :301). */

@@ -59,22 +59,36 @@ enum Dtype { DT_F32 = 0, DT_F16 = 1, DT_BF16 = 2, DT_F64 = 3 };
 
 struct bf16_t { uint16_t bits; };
 
-// Storage policy: S is the element type in HBM, T the arithmetic type.  The 16-bit
-// forms store (f_i - w_i) * scale; scale is a power of two so the scaling is exact.
+// Storage policy: S is the element type in HBM, T the arithmetic type, B the raw bits.
+// The 16-bit forms store (f_i - w_i) * scale; scale is a power of two so the scaling is
+// exact.  decode/encode convert between raw bits and arithmetic values.
 template <class S> struct Codec;
 
 template <> struct Codec<float> {
     using T = float;
+    using B = uint32_t;
+    static __device__ __forceinline__ float decode(B b, float, float) { return __uint_as_float(b); }
+    static __device__ __forceinline__ B encode(float v, float, float) { return __float_as_uint(v); }
     static __device__ __forceinline__ float load(const float* p, size_t i, float, float) { return p[i]; }
     static __device__ __forceinline__ void store(float* p, size_t i, float v, float, float) { p[i] = v; }
 };
 template <> struct Codec<double> {
     using T = double;
+    using B = uint64_t;
+    static __device__ __forceinline__ double decode(B b, double, double) { return __builtin_bit_cast(double, b); }
+    static __device__ __forceinline__ B encode(double v, double, double) { return __builtin_bit_cast(B, v); }
     static __device__ __forceinline__ double load(const double* p, size_t i, double, double) { return p[i]; }
     static __device__ __forceinline__ void store(double* p, size_t i, double v, double, double) { p[i] = v; }
 };
 template <> struct Codec<_Float16> {
     using T = float;
+    using B = uint16_t;
+    static __device__ __forceinline__ float decode(B b, float w, float inv_scale) {
+        return w + (float)__builtin_bit_cast(_Float16, b) * inv_scale;
+    }
+    static __device__ __forceinline__ B encode(float v, float w, float scale) {
+        return __builtin_bit_cast(B, (_Float16)((v - w) * scale));
+    }
     static __device__ __forceinline__ float load(const _Float16* p, size_t i, float w, float inv_scale) {
         return w + (float)p[i] * inv_scale;
     }
@@ -91,11 +105,69 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest
 }
 template <> struct Codec<bf16_t> {
     using T = float;
+    using B = uint16_t;
+    static __device__ __forceinline__ float decode(B b, float w, float inv_scale) { return w + bf16_to_f32(b) * inv_scale; }
+    static __device__ __forceinline__ B encode(float v, float w, float scale) { return f32_to_bf16((v - w) * scale); }
     static __device__ __forceinline__ float load(const bf16_t* p, size_t i, float w, float inv_scale) {
         return w + bf16_to_f32(p[i].bits) * inv_scale;
     }
     static __device__ __forceinline__ void store(bf16_t* p, size_t i, float v, float w, float scale) {
         p[i].bits = f32_to_bf16((v - w) * scale);
+    }
+};
+
+// ------------------------------------------------------------- memory access
+// Raw-buffer access: one descriptor per array, a wave-uniform 32-bit byte offset (row
+// start, scalar register) plus a per-lane 32-bit byte offset.  Usable while every byte
+// offset of the array fits in 32 bits.
+constexpr int BUF_FLAGS = 0x00020000;   // raw buffer, 32-bit data format (gfx9 family)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, BUF_FLAGS);
+}
+
+template <class B, int AUX = 0>
+__device__ __forceinline__ B buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(B) == 2) return (B)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, AUX);
+    else if constexpr (sizeof(B) == 4) return (B)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX);
+    else return __builtin_bit_cast(B, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+}
+template <class B, int AUX = 0>
+__device__ __forceinline__ void buf_store(B v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(B) == 2) __builtin_amdgcn_raw_buffer_store_b16(v, r, voff, soff, AUX);
+    else if constexpr (sizeof(B) == 4) __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, AUX);
+    else {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, AUX);
+    }
+}
+
+// Accessor over one population array pair.  BUF = true: raw buffers (offsets < 4 GiB);
+// BUF = false: 64-bit pointers for arrays larger than that.
+template <class S, bool BUF> struct PopMem;
+
+template <class S> struct PopMem<S, true> {
+    using B = typename Codec<S>::B;
+    __amdgpu_buffer_rsrc_t rin, rout;
+    __device__ __forceinline__ PopMem(const void* fin, void* fout) : rin(make_rsrc(fin)), rout(make_rsrc(fout)) {}
+    __device__ __forceinline__ B ld(size_t row_bytes, uint32_t lane_bytes) const {
+        return buf_load<B>(rin, lane_bytes, (uint32_t)row_bytes);
+    }
+    __device__ __forceinline__ void st(size_t row_bytes, uint32_t lane_bytes, B v) const {
+        buf_store<B>(v, rout, lane_bytes, (uint32_t)row_bytes);
+    }
+};
+template <class S> struct PopMem<S, false> {
+    using B = typename Codec<S>::B;
+    const char* in;
+    char* out;
+    __device__ __forceinline__ PopMem(const void* fin, void* fout)
+        : in(static_cast<const char*>(fin)), out(static_cast<char*>(fout)) {}
+    __device__ __forceinline__ B ld(size_t row_bytes, uint32_t lane_bytes) const {
+        return *reinterpret_cast<const B*>(in + row_bytes + lane_bytes);
+    }
+    __device__ __forceinline__ void st(size_t row_bytes, uint32_t lane_bytes, B v) const {
+        *reinterpret_cast<B*>(out + row_bytes + lane_bytes) = v;
     }
 };
 

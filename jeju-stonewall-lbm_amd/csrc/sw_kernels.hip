@@ -5,15 +5,17 @@
 namespace sw {
 
 // ------------------------------------------------------------------ step
-template <class S, int YBC>
+template <class S, int YBC, int CPL, bool BUF>
 __global__ __launch_bounds__(WG) void k_step(StepArgs a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform per wavefront; readfirstlane lets the compiler keep every
+    // row-derived pointer in scalar registers
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int z = blockIdx.x / a.nyb;
     const int yb = blockIdx.x - z * a.nyb;
     const int y = yb * ROWS + wave;
     const int t = a.t + (a.t_dev ? *a.t_dev : 0);
     double fx = 0.0, fz = 0.0;
-    if (y < a.ny) step_row<S, YBC>(a, y, z, t, lane, fx, fz);
+    if (y < a.ny) step_row<S, YBC, CPL, BUF>(a, y, z, t, lane, fx, fz);
     block_force(a, fx, fz, z, yb, t);
 }
 
@@ -169,11 +171,31 @@ __global__ void k_transpose(const E* __restrict__ src, E* __restrict__ dst, int 
 
 using namespace sw;
 
-template <class S>
-static void launch_step(const StepArgs& a, int ybc, hipStream_t s) {
+// Step variants (the autotuner's choices, per dtype): 0 = one 64-cell chunk per
+// wavefront iteration, 1 = two chunks per iteration (more loads in flight, fewer waves).
+constexpr int SW_STEP_VARIANTS = 2;
+static int g_step_variant[4] = {0, 0, 0, 0};
+
+template <class S, int CPL, bool BUF>
+static void launch_step_v(const StepArgs& a, int ybc, hipStream_t s) {
     dim3 grid(a.nz * a.nyb), blk(WG);
-    if (ybc == Y_PERIODIC) hipLaunchKernelGGL((k_step<S, Y_PERIODIC>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_step<S, Y_FREESLIP>), grid, blk, 0, s, a);
+    if (ybc == Y_PERIODIC) hipLaunchKernelGGL((k_step<S, Y_PERIODIC, CPL, BUF>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_step<S, Y_FREESLIP, CPL, BUF>), grid, blk, 0, s, a);
+}
+
+// raw-buffer addressing needs every byte offset of a population array below 4 GiB
+static bool fits_buffer(const StepArgs& a, size_t elem) {
+    return (size_t)Q * a.nx * a.ny * a.nz * elem < ((size_t)1 << 32);
+}
+
+template <class S>
+static void launch_step(const StepArgs& a, int ybc, int dtype, hipStream_t s) {
+    const bool buf = fits_buffer(a, sizeof(S));
+    if (g_step_variant[dtype] == 1) {
+        if (buf) launch_step_v<S, 2, true>(a, ybc, s); else launch_step_v<S, 2, false>(a, ybc, s);
+    } else {
+        if (buf) launch_step_v<S, 1, true>(a, ybc, s); else launch_step_v<S, 1, false>(a, ybc, s);
+    }
 }
 
 
@@ -251,14 +273,25 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
     a.scale = scale; a.inv_scale = 1.0f / scale;
     hipStream_t s = (hipStream_t)stream;
     switch (dtype) {
-        case DT_F32: launch_step<float>(a, ybc, s); break;
-        case DT_F64: launch_step<double>(a, ybc, s); break;
-        case DT_F16: launch_step<_Float16>(a, ybc, s); break;
-        case DT_BF16: launch_step<bf16_t>(a, ybc, s); break;
+        case DT_F32: launch_step<float>(a, ybc, dtype, s); break;
+        case DT_F64: launch_step<double>(a, ybc, dtype, s); break;
+        case DT_F16: launch_step<_Float16>(a, ybc, dtype, s); break;
+        case DT_BF16: launch_step<bf16_t>(a, ybc, dtype, s); break;
     }
     SW_HIP(hipGetLastError());
     return SW_OK;
 }
+
+int sw_step_variants(void) { return SW_STEP_VARIANTS; }
+
+int sw_set_step_variant(int dtype, int variant) {
+    SW_CHECK_ARG(dtype >= 0 && dtype <= 3, "dtype must be 0..3");
+    SW_CHECK_ARG(variant >= 0 && variant < SW_STEP_VARIANTS, "unknown step variant");
+    g_step_variant[dtype] = variant;
+    return SW_OK;
+}
+
+int sw_get_step_variant(int dtype) { return (dtype >= 0 && dtype <= 3) ? g_step_variant[dtype] : -1; }
 
 int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int ny, int nz,
                     double* force, void* stream) {

@@ -27,6 +27,9 @@ SIGNATURES = {
     "sw_init": (_c_int, [_p, _c_int, _p, _p, _c_int, _c_int, _c_int, _c_flt, _c_uint, _c_flt, _p]),
     "sw_step": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
                          _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _c_int, _p]),
+    "sw_step_variants": (_c_int, []),
+    "sw_set_step_variant": (_c_int, [_c_int, _c_int]),
+    "sw_get_step_variant": (_c_int, [_c_int]),
     "sw_force_reduce": (_c_int, [_p, _c_int, _c_int, _p, _c_int, _c_int, _c_int, _p, _p]),
     "sw_macro": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_flt, _p, _p, _p]),
     "sw_to_ref_layout": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
