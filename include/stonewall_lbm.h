@@ -53,6 +53,10 @@ const char* sw_last_error(void);
 /* Rows (y lines) per step workgroup; the partials ring holds ceil(ny/rows) pairs per z. */
 int sw_rows_per_block(void);
 
+/* uint32 words of a links buffer: nx*ny*nz link masks followed by per-row chunk flags
+ * (one bit per 64-cell chunk that touches a solid). */
+long long sw_links_words(int nx, int ny, int nz);
+
 /* Doubles per step in the force-partials ring: nz * ceil(ny/rows) * 2. */
 long long sw_partials_per_step(int ny, int nz);
 
@@ -63,7 +67,7 @@ long long sw_partials_per_step(int ny, int nz);
  *   flag  : (nx, ny, nz) uint8, z contiguous, values FLUID/WALL/GROUND as lbm3d.py:30;
  *           the layer z=0 must be solid, and flag >= 2 may only be reached through
  *           links with cz = +1 (the ground), else SW_ERR_GEOMETRY.
- *   links : [nz][ny][nx] uint32 output.
+ *   links : sw_links_words(nx, ny, nz) uint32 output: [nz][ny][nx] masks, then chunk flags.
  *   ybc   : SW_Y_PERIODIC (the reference, lbm3d.py:79) or SW_Y_FREESLIP.
  * Synchronizes `stream` (it reports geometry errors). */
 int sw_prepare_links(const uint8_t* flag, uint32_t* links, int nx, int ny, int nz, int ybc, void* stream);
@@ -91,7 +95,9 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
 
 /* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
  * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept
- * per dtype for the process: 0 = one 64-cell chunk per wavefront iteration, 1 = two. */
+ * per dtype for the process: 0 = one 64-cell chunk per wavefront iteration, 1 = two,
+ * 2 = one with the next chunk's loads issued first, 3 = 2 + nontemporal stores,
+ * 4 = 0 + nontemporal stores, 5 = 1 + nontemporal stores. */
 int sw_step_variants(void);
 int sw_set_step_variant(int dtype, int variant);
 int sw_get_step_variant(int dtype);
@@ -107,6 +113,49 @@ int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, 
  * Solid cells give (1, 0, 0, 0). */
 int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale, float* U,
              float* Uacc, void* stream);
+
+/* ---------------------------------------------------------------- batched form
+ * No reference counterpart: the reference runs one lattice per process (run_all.sh).
+ * Many independent lattices of one dtype advance together: one persistent launch per
+ * step strides over the row-groups of every lattice in a device-resident table.  Each
+ * lattice's arithmetic is that of sw_step, so its results are bit-identical alone and
+ * in any mix.  The table is rewritten between steps when lattices join or leave. */
+typedef struct sw_instance {
+    void* f0;                 /* population buffer that holds step t when parity == 0 */
+    void* f1;                 /* the other buffer */
+    const uint32_t* links;    /* from sw_prepare_links */
+    const float* uin;         /* [nz] inlet profile */
+    const float* gust;        /* [t_end] inlet factor per step, or NULL */
+    double* partials;         /* ring * sw_partials_per_step(ny, nz) doubles, or NULL */
+    double* force;            /* [t_end][nz][2] force record, or NULL */
+    double nu0, cs, nu_sponge;
+    float scale;
+    int nx, ny, nz;
+    int ring, x_sponge, ybc, dtype;
+    int t, t_end, parity;     /* next step, end step (exclusive), buffer of step t */
+    int item0;                /* first work item; written by sw_batch_upload */
+    int reserved;
+} sw_instance;
+
+/* Bytes of a device table holding up to max_instances lattices. */
+long long sw_batch_table_bytes(int max_instances);
+/* Writes n instances (item0 computed here) into the device table; synchronizes stream. */
+int sw_batch_upload(const sw_instance* instances, int n, int max_instances, void* table, void* stream);
+/* Copies the table's n entries back (t, parity, ...); synchronizes stream. */
+int sw_batch_download(const void* table, sw_instance* instances, int n, void* stream);
+/* Step t_i + k of every unfinished instance; `blocks` persistent workgroups; variant as
+ * sw_set_step_variant; big_offsets = 1 when some population array exceeds 4 GiB. */
+int sw_batch_step(const void* table, int dtype, int k, int blocks, int variant, int big_offsets, void* stream);
+/* Force rows of steps t_i .. t_i + K - 1 (before sw_batch_advance); needs ring >= K. */
+int sw_batch_force_reduce(const void* table, int max_instances, int K, void* stream);
+/* t_i += K (clamped to t_end) and the matching parity flip. */
+int sw_batch_advance(void* table, int max_instances, int K, void* stream);
+/* Captures K batch steps + force reduction + advance as one hipGraph (graph handle out);
+ * replay with sw_graph_launch on any stream, free with sw_graph_destroy. */
+int sw_batch_graph_create(void* table, int max_instances, int dtype, int K, int blocks, int variant, int big_offsets,
+                          void** graph);
+int sw_graph_launch(void* graph, void* stream);
+int sw_graph_destroy(void* graph);
 
 /* Layout conversion of ncomp stacked fields between [c][z][y][x] (this library) and
  * [c][x][y][z] (the reference's arrays, lbm3d.py:293-303, used by its checkpoint files,

@@ -1,0 +1,278 @@
+// Batched form: many independent lattices of uneven size advanced by one launch per step.
+//
+// A device-resident instance table (header + sw_instance entries, include/stonewall_lbm.h)
+// describes every lattice: its two population buffers, link masks, inlet, gust table,
+// force-partials ring, force record, physics scalars, its next step t and end step
+// t_end, and which buffer holds step t (parity).  The work items of a batch are the
+// row-groups (ROWS rows at one z) of all lattices, numbered instance by instance
+// (item0 = prefix sum).  k_batch_step is persistent: a fixed grid of workgroups strides
+// over the items, so one launch configuration (and one captured hipGraph) serves any
+// composition of lattices; the composition lives only in the table, which the host
+// rewrites between graph replays when lattices are admitted or retired.
+//
+// Each lattice's arithmetic and force reduction are exactly those of the single-lattice
+// kernel (same step_row, same per-workgroup partials, same fixed-order reduction), so a
+// lattice's results are bit-identical alone and in any mix.
+#include "sw_step.hpp"
+#include "sw_api_util.hpp"
+
+namespace sw {
+
+struct TableHeader {
+    int n;              // instances in the table
+    int total_items;    // sum of nz * ceil(ny / ROWS)
+    int max_instances;  // capacity
+    int pad[13];
+};
+static_assert(sizeof(TableHeader) == 64, "header is 64 bytes");
+static_assert(sizeof(sw_instance) % 8 == 0, "instance entries stay 8-byte aligned");
+
+__device__ __forceinline__ const sw_instance* table_entries(const void* table) {
+    return reinterpret_cast<const sw_instance*>(static_cast<const char*>(table) + sizeof(TableHeader));
+}
+
+// instance owning work item `item` (item0 ascending); all arguments wave-uniform
+__device__ __forceinline__ int find_instance(const sw_instance* e, int n, int item) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (e[mid].item0 <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ StepArgs instance_args(const sw_instance& I, int k) {
+    StepArgs a{};
+    const bool odd = ((I.parity + k) & 1) != 0;
+    a.f_in = odd ? I.f1 : I.f0;
+    a.f_out = odd ? I.f0 : I.f1;
+    a.links = I.links; a.uin = I.uin; a.gust = I.gust; a.partials = I.partials; a.t_dev = nullptr;
+    a.ring = I.ring; a.nx = I.nx; a.ny = I.ny; a.nz = I.nz; a.nyb = (I.ny + ROWS - 1) / ROWS;
+    a.x_sponge = I.x_sponge; a.ybc = I.ybc; a.nu0 = I.nu0; a.cs = I.cs; a.nu_sponge = I.nu_sponge;
+    a.scale = I.scale; a.inv_scale = 1.0f / I.scale;
+    return a;
+}
+
+template <class S, class TL, bool BUF>
+__global__ __launch_bounds__(64 * TL::WPB) void k_batch_step(const void* table, int k) {
+    const TableHeader* h = static_cast<const TableHeader*>(table);
+    const sw_instance* e = table_entries(table);
+    const int n = h->n, total = h->total_items;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int i = find_instance(e, n, item);
+        const sw_instance& I = e[i];
+        const int t = I.t + k;
+        if (t >= I.t_end) continue;                          // finished: uniform over the workgroup
+        const StepArgs a = instance_args(I, k);
+        const int local = item - I.item0;
+        const int z = local / a.nyb, yb = local - z * a.nyb;
+        const int y = yb * ROWS + (wave % ROWS);
+        double fx = 0.0, fz = 0.0;
+        if (y < a.ny) step_row<S, TL, BUF>(a, y, z, t, lane, wave / ROWS, fx, fz);
+        block_force<TL::WPB>(a, fx, fz, z, yb, t);
+    }
+}
+
+// force rows of steps t .. min(t + K, t_end) - 1 of every instance (before the advance)
+__global__ __launch_bounds__(256) void k_batch_force(const void* table, int K) {
+    const TableHeader* h = static_cast<const TableHeader*>(table);
+    const int i = blockIdx.x, s = blockIdx.y;
+    if (i >= h->n) return;
+    const sw_instance& I = table_entries(table)[i];
+    const int t = I.t + s;
+    if (t >= I.t_end || s >= I.ring || !I.partials || !I.force) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nyb = (I.ny + ROWS - 1) / ROWS;
+    for (int z = wave; z < I.nz; z += 4) {
+        const double* p = I.partials + (((size_t)(t % I.ring) * I.nz + z) * nyb) * 2;
+        double sx = 0.0, sz = 0.0;
+        for (int q = lane; q < nyb; q += 64) { sx += p[2 * q]; sz += p[2 * q + 1]; }
+        sx = wave_sum(sx);
+        sz = wave_sum(sz);
+        if (lane == 0) {
+            I.force[((size_t)t * I.nz + z) * 2 + 0] = sx;
+            I.force[((size_t)t * I.nz + z) * 2 + 1] = sz;
+        }
+    }
+}
+
+__global__ void k_batch_advance(void* table, int K) {
+    TableHeader* h = static_cast<TableHeader*>(table);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h->n) return;
+    sw_instance* e = reinterpret_cast<sw_instance*>(static_cast<char*>(table) + sizeof(TableHeader));
+    const int t0 = e[i].t;
+    const int t1 = min(t0 + K, e[i].t_end);
+    e[i].parity = (e[i].parity + (t1 - t0)) & 1;
+    e[i].t = t1;
+}
+
+// -------------------------------------------------------------------- host side
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+template <class S, class TL, bool BUF>
+static void launch_batch_v(const void* table, int k, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_batch_step<S, TL, BUF>), dim3(blocks), dim3(64 * TL::WPB), 0, s, table, k);
+}
+
+// variants as sw_set_step_variant; lattices with > 4 GiB population arrays use 64-bit
+// addressing and variant 0
+template <class S>
+static void launch_batch(const void* table, int k, int blocks, int variant, bool big, hipStream_t s) {
+    if (big) { launch_batch_v<S, TileV0, false>(table, k, blocks, s); return; }
+    switch (variant) {
+#define SW_CASE(id, TL) case id: launch_batch_v<S, TL, true>(table, k, blocks, s); break;
+        SW_TILE_LIST(SW_CASE)
+#undef SW_CASE
+        default: launch_batch_v<S, TileV1, true>(table, k, blocks, s); break;
+    }
+}
+
+static int batch_step_launch(const void* table, int dtype, int k, int blocks, int variant, int big, hipStream_t s) {
+    switch (dtype) {
+        case DT_F32: launch_batch<float>(table, k, blocks, variant, big, s); break;
+        case DT_F64: launch_batch<double>(table, k, blocks, variant, big, s); break;
+        case DT_F16: launch_batch<_Float16>(table, k, blocks, variant, big, s); break;
+        case DT_BF16: launch_batch<bf16_t>(table, k, blocks, variant, big, s); break;
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace sw
+
+using namespace sw;
+
+extern "C" {
+
+long long sw_batch_table_bytes(int max_instances) {
+    if (max_instances <= 0) return -1;
+    return (long long)sizeof(TableHeader) + (long long)max_instances * (long long)sizeof(sw_instance);
+}
+
+int sw_batch_upload(const sw_instance* instances, int n, int max_instances, void* table, void* stream) {
+    SW_CHECK_ARG(table && (n == 0 || instances), "table and instances must be non-null");
+    SW_CHECK_ARG(n >= 0 && n <= max_instances, "need 0 <= n <= max_instances");
+    const size_t bytes = sizeof(TableHeader) + (size_t)n * sizeof(sw_instance);
+    char* host = (char*)malloc(bytes);
+    if (!host) return set_error(SW_ERR_CAPACITY, "sw_batch_upload: host allocation failed");
+    TableHeader h{};
+    h.n = n;
+    h.max_instances = max_instances;
+    sw_instance* e = (sw_instance*)(host + sizeof(TableHeader));
+    long long items = 0;
+    for (int i = 0; i < n; i++) {
+        e[i] = instances[i];
+        const sw_instance& I = e[i];
+        int rc = check_dims(I.nx, I.ny, I.nz);
+        if (rc) { free(host); return rc; }
+        if (!I.f0 || !I.f1 || !I.links || !I.uin || I.f0 == I.f1) {
+            free(host);
+            return set_error(SW_ERR_INVALID, "sw_batch_upload: instance buffers missing or aliased");
+        }
+        if (I.dtype < 0 || I.dtype > 3 || check_dtype(I.dtype, I.scale)) {
+            free(host);
+            return set_error(SW_ERR_INVALID, "sw_batch_upload: bad instance dtype / scale");
+        }
+        if ((I.partials && I.ring <= 0) || I.t < 0 || I.t > I.t_end || (I.ybc != 0 && I.ybc != 1)) {
+            free(host);
+            return set_error(SW_ERR_INVALID, "sw_batch_upload: bad ring / t / t_end / ybc");
+        }
+        e[i].item0 = (int)items;
+        items += (long long)I.nz * ((I.ny + ROWS - 1) / ROWS);
+    }
+    if (items > 0x7FFFFFFFLL) { free(host); return set_error(SW_ERR_TOO_LARGE, "sw_batch_upload: too many rows"); }
+    h.total_items = (int)items;
+    memcpy(host, &h, sizeof(h));
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t err = hipMemcpyAsync(table, host, bytes, hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    free(host);
+    if (err != hipSuccess) return set_hip_error(err, "sw_batch_upload");
+    return SW_OK;
+}
+
+int sw_batch_download(const void* table, sw_instance* instances, int n, void* stream) {
+    SW_CHECK_ARG(table && (n == 0 || instances) && n >= 0, "table and instances must be non-null");
+    hipStream_t s = (hipStream_t)stream;
+    SW_HIP(hipMemcpyAsync(instances, (const char*)table + sizeof(TableHeader), (size_t)n * sizeof(sw_instance),
+                          hipMemcpyDeviceToHost, s));
+    SW_HIP(hipStreamSynchronize(s));
+    return SW_OK;
+}
+
+int sw_batch_step(const void* table, int dtype, int k, int blocks, int variant, int big_offsets, void* stream) {
+    SW_CHECK_ARG(table, "table must be non-null");
+    SW_CHECK_ARG(dtype >= 0 && dtype <= 3, "dtype must be 0..3");
+    SW_CHECK_ARG(blocks > 0 && k >= 0 && variant >= 0 && variant < NUM_TILE_VARIANTS, "need blocks > 0, k >= 0, variant 0..7");
+    int rc = batch_step_launch(table, dtype, k, blocks, variant, big_offsets != 0, (hipStream_t)stream);
+    if (rc) return set_hip_error((hipError_t)rc, "sw_batch_step");
+    return SW_OK;
+}
+
+int sw_batch_force_reduce(const void* table, int max_instances, int K, void* stream) {
+    SW_CHECK_ARG(table && max_instances > 0 && K > 0 && K <= 65535, "bad table / max_instances / K");
+    hipLaunchKernelGGL(k_batch_force, dim3(max_instances, K), dim3(256), 0, (hipStream_t)stream, table, K);
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_batch_advance(void* table, int max_instances, int K, void* stream) {
+    SW_CHECK_ARG(table && max_instances > 0 && K > 0, "bad table / max_instances / K");
+    hipLaunchKernelGGL(k_batch_advance, dim3((max_instances + 255) / 256), dim3(256), 0, (hipStream_t)stream, table,
+                       K);
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_batch_graph_create(void* table, int max_instances, int dtype, int K, int blocks, int variant, int big_offsets,
+                          void** graph) {
+    SW_CHECK_ARG(table && graph && max_instances > 0 && K > 0 && blocks > 0, "bad arguments");
+    SW_CHECK_ARG(dtype >= 0 && dtype <= 3 && variant >= 0 && variant < NUM_TILE_VARIANTS, "bad dtype / variant");
+    *graph = nullptr;
+    hipStream_t cs = nullptr;
+    SW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    BatchGraph* g = new BatchGraph();
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = 0;
+    if (e == hipSuccess) {
+        for (int k = 0; k < K && !rc; k++) rc = batch_step_launch(table, dtype, k, blocks, variant, big_offsets != 0, cs);
+        if (!rc) {
+            hipLaunchKernelGGL(k_batch_force, dim3(max_instances, K), dim3(256), 0, cs, (const void*)table, K);
+            hipLaunchKernelGGL(k_batch_advance, dim3((max_instances + 255) / 256), dim3(256), 0, cs, table, K);
+            rc = (int)hipGetLastError();
+        }
+        hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+        if (e == hipSuccess) e = e2;
+    }
+    if (e == hipSuccess && rc == 0) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess || rc != 0) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return set_hip_error(e != hipSuccess ? e : (hipError_t)rc, "sw_batch_graph_create");
+    }
+    *graph = g;
+    return SW_OK;
+}
+
+int sw_graph_launch(void* graph, void* stream) {
+    SW_CHECK_ARG(graph, "graph must be non-null");
+    SW_HIP(hipGraphLaunch(static_cast<BatchGraph*>(graph)->exec, (hipStream_t)stream));
+    return SW_OK;
+}
+
+int sw_graph_destroy(void* graph) {
+    if (!graph) return SW_OK;
+    BatchGraph* g = static_cast<BatchGraph*>(graph);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return SW_OK;
+}
+
+}  // extern "C"

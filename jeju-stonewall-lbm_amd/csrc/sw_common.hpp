@@ -83,11 +83,13 @@ template <> struct Codec<double> {
 template <> struct Codec<_Float16> {
     using T = float;
     using B = uint16_t;
+    // w + h*2^-k with one rounding: h*2^-k is exact, so fma == mul-then-add bit for bit
     static __device__ __forceinline__ float decode(B b, float w, float inv_scale) {
-        return w + (float)__builtin_bit_cast(_Float16, b) * inv_scale;
+        return __builtin_fmaf((float)__builtin_bit_cast(_Float16, b), inv_scale, w);
     }
+    // RN(v - w) * 2^k == RN(v*2^k - w*2^k): scaling by a power of two commutes with rounding
     static __device__ __forceinline__ B encode(float v, float w, float scale) {
-        return __builtin_bit_cast(B, (_Float16)((v - w) * scale));
+        return __builtin_bit_cast(B, (_Float16)__builtin_fmaf(v, scale, -(w * scale)));
     }
     static __device__ __forceinline__ float load(const _Float16* p, size_t i, float w, float inv_scale) {
         return w + (float)p[i] * inv_scale;
@@ -97,17 +99,18 @@ template <> struct Codec<_Float16> {
     }
 };
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest even (hardware convert)
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 template <> struct Codec<bf16_t> {
     using T = float;
     using B = uint16_t;
-    static __device__ __forceinline__ float decode(B b, float w, float inv_scale) { return w + bf16_to_f32(b) * inv_scale; }
-    static __device__ __forceinline__ B encode(float v, float w, float scale) { return f32_to_bf16((v - w) * scale); }
+    static __device__ __forceinline__ float decode(B b, float w, float inv_scale) {
+        return __builtin_fmaf(bf16_to_f32(b), inv_scale, w);
+    }
+    static __device__ __forceinline__ B encode(float v, float w, float scale) {
+        return f32_to_bf16(__builtin_fmaf(v, scale, -(w * scale)));
+    }
     static __device__ __forceinline__ float load(const bf16_t* p, size_t i, float w, float inv_scale) {
         return w + bf16_to_f32(p[i].bits) * inv_scale;
     }
@@ -150,11 +153,13 @@ template <class S> struct PopMem<S, true> {
     using B = typename Codec<S>::B;
     __amdgpu_buffer_rsrc_t rin, rout;
     __device__ __forceinline__ PopMem(const void* fin, void* fout) : rin(make_rsrc(fin)), rout(make_rsrc(fout)) {}
+    template <bool NT = false>
     __device__ __forceinline__ B ld(size_t row_bytes, uint32_t lane_bytes) const {
-        return buf_load<B>(rin, lane_bytes, (uint32_t)row_bytes);
+        return buf_load<B, NT ? 2 : 0>(rin, lane_bytes, (uint32_t)row_bytes);
     }
+    template <bool NT = false>
     __device__ __forceinline__ void st(size_t row_bytes, uint32_t lane_bytes, B v) const {
-        buf_store<B>(v, rout, lane_bytes, (uint32_t)row_bytes);
+        buf_store<B, NT ? 2 : 0>(v, rout, lane_bytes, (uint32_t)row_bytes);
     }
 };
 template <class S> struct PopMem<S, false> {
@@ -163,11 +168,15 @@ template <class S> struct PopMem<S, false> {
     char* out;
     __device__ __forceinline__ PopMem(const void* fin, void* fout)
         : in(static_cast<const char*>(fin)), out(static_cast<char*>(fout)) {}
+    template <bool NT = false>
     __device__ __forceinline__ B ld(size_t row_bytes, uint32_t lane_bytes) const {
-        return *reinterpret_cast<const B*>(in + row_bytes + lane_bytes);
+        const B* p = reinterpret_cast<const B*>(in + row_bytes + lane_bytes);
+        if constexpr (NT) return __builtin_nontemporal_load(p); else return *p;
     }
+    template <bool NT = false>
     __device__ __forceinline__ void st(size_t row_bytes, uint32_t lane_bytes, B v) const {
-        *reinterpret_cast<B*>(out + row_bytes + lane_bytes) = v;
+        B* p = reinterpret_cast<B*>(out + row_bytes + lane_bytes);
+        if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
     }
 };
 

@@ -5,18 +5,18 @@
 namespace sw {
 
 // ------------------------------------------------------------------ step
-template <class S, int YBC, int CPL, bool BUF>
-__global__ __launch_bounds__(WG) void k_step(StepArgs a) {
+template <class S, class TL, bool BUF>
+__global__ __launch_bounds__(64 * TL::WPB) void k_step(StepArgs a) {
     // the wave index is uniform per wavefront; readfirstlane lets the compiler keep every
     // row-derived pointer in scalar registers
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int z = blockIdx.x / a.nyb;
     const int yb = blockIdx.x - z * a.nyb;
-    const int y = yb * ROWS + wave;
+    const int y = yb * ROWS + (wave % ROWS);
     const int t = a.t + (a.t_dev ? *a.t_dev : 0);
     double fx = 0.0, fz = 0.0;
-    if (y < a.ny) step_row<S, YBC, CPL, BUF>(a, y, z, t, lane, fx, fz);
-    block_force(a, fx, fz, z, yb, t);
+    if (y < a.ny) step_row<S, TL, BUF>(a, y, z, t, lane, wave / ROWS, fx, fz);
+    block_force<TL::WPB>(a, fx, fz, z, yb, t);
 }
 
 // ------------------------------------------------------------------ link masks
@@ -50,6 +50,24 @@ __global__ void k_prepare(const uint8_t* __restrict__ flag, uint32_t* __restrict
         }
     }
     links[((size_t)z * ny + y) * nx + x] = m;
+}
+
+// chunk flags: one thread per (row, 32-chunk word); bit c set when 64-cell chunk c of
+// the row holds a solid cell or a cell with a solid neighbour (the inlet cell excluded)
+__global__ void k_chunk_flags(uint32_t* __restrict__ links, int nx, int ny, int nz) {
+    const uint32_t cw = chunk_words(nx);
+    const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (gid >= (size_t)ny * nz * cw) return;
+    const uint32_t row = (uint32_t)(gid / cw), word = (uint32_t)(gid % cw);
+    const uint32_t* lrow = links + (size_t)row * nx;
+    uint32_t bits = 0;
+    for (uint32_t c = 0; c < 32; c++) {
+        const uint32_t x0 = (word * 32 + c) * 64;
+        if (x0 >= (uint32_t)nx) break;
+        for (uint32_t x = x0; x < x0 + 64 && x < (uint32_t)nx; x++)
+            if (x != 0 && lrow[x] != 0u) { bits |= 1u << c; break; }
+    }
+    links[(size_t)nx * ny * nz + gid] = bits;
 }
 
 // ------------------------------------------------------------------ init (lbm3d.py:39-54)
@@ -171,16 +189,14 @@ __global__ void k_transpose(const E* __restrict__ src, E* __restrict__ dst, int 
 
 using namespace sw;
 
-// Step variants (the autotuner's choices, per dtype): 0 = one 64-cell chunk per
-// wavefront iteration, 1 = two chunks per iteration (more loads in flight, fewer waves).
-constexpr int SW_STEP_VARIANTS = 2;
-static int g_step_variant[4] = {0, 0, 0, 0};
+// Step variants (the autotuner's choices, per dtype): the tile list of sw_step.hpp.
+// Lattices whose byte offsets exceed 32 bits always use variant 0 with 64-bit addressing.
+static int g_step_variant[4] = {7, 1, 1, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
 
-template <class S, int CPL, bool BUF>
-static void launch_step_v(const StepArgs& a, int ybc, hipStream_t s) {
-    dim3 grid(a.nz * a.nyb), blk(WG);
-    if (ybc == Y_PERIODIC) hipLaunchKernelGGL((k_step<S, Y_PERIODIC, CPL, BUF>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_step<S, Y_FREESLIP, CPL, BUF>), grid, blk, 0, s, a);
+template <class S, class TL, bool BUF>
+static void launch_step_v(const StepArgs& a, hipStream_t s) {
+    dim3 grid(a.nz * a.nyb), blk(64 * TL::WPB);
+    hipLaunchKernelGGL((k_step<S, TL, BUF>), grid, blk, 0, s, a);
 }
 
 // raw-buffer addressing needs every byte offset of a population array below 4 GiB
@@ -189,17 +205,16 @@ static bool fits_buffer(const StepArgs& a, size_t elem) {
 }
 
 template <class S>
-static void launch_step(const StepArgs& a, int ybc, int dtype, hipStream_t s) {
-    const bool buf = fits_buffer(a, sizeof(S));
-    if (g_step_variant[dtype] == 1) {
-        if (buf) launch_step_v<S, 2, true>(a, ybc, s); else launch_step_v<S, 2, false>(a, ybc, s);
-    } else {
-        if (buf) launch_step_v<S, 1, true>(a, ybc, s); else launch_step_v<S, 1, false>(a, ybc, s);
+static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
+    if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV0, false>(a, s); return; }
+    switch (g_step_variant[dtype]) {
+#define SW_CASE(id, TL) case id: launch_step_v<S, TL, true>(a, s); break;
+        SW_TILE_LIST(SW_CASE)
+#undef SW_CASE
+        default: launch_step_v<S, TileV1, true>(a, s); break;
     }
 }
 
-
-// =================================================================== C ABI
 extern "C" {
 
 int sw_version(void) { return SW_VERSION; }
@@ -207,6 +222,11 @@ int sw_version(void) { return SW_VERSION; }
 const char* sw_last_error(void) { return last_error_message(); }
 
 int sw_rows_per_block(void) { return ROWS; }
+
+long long sw_links_words(int nx, int ny, int nz) {
+    if (check_dims(nx, ny, nz)) return -1;
+    return (long long)nx * ny * nz + (long long)ny * nz * chunk_words(nx);
+}
 
 long long sw_partials_per_step(int ny, int nz) {
     if (ny <= 0 || nz <= 0) return -1;
@@ -224,6 +244,8 @@ int sw_prepare_links(const uint8_t* flag, uint32_t* links, int nx, int ny, int n
     hipError_t e = hipMemsetAsync(err, 0, sizeof(int), s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_prepare, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, s, flag, links, nx, ny, nz, ybc, err);
+        const size_t nthreads = (size_t)ny * nz * chunk_words(nx);
+        hipLaunchKernelGGL(k_chunk_flags, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, links, nx, ny, nz);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -269,24 +291,24 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
     a.f_in = f_in; a.f_out = f_out; a.links = links; a.uin = uin; a.gust = gust;
     a.partials = partials; a.t_dev = t_dev; a.t = t; a.ring = ring > 0 ? ring : 1;
     a.nx = nx; a.ny = ny; a.nz = nz; a.nyb = (ny + ROWS - 1) / ROWS;
-    a.x_sponge = x_sponge; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
+    a.x_sponge = x_sponge; a.ybc = ybc; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
     a.scale = scale; a.inv_scale = 1.0f / scale;
     hipStream_t s = (hipStream_t)stream;
     switch (dtype) {
-        case DT_F32: launch_step<float>(a, ybc, dtype, s); break;
-        case DT_F64: launch_step<double>(a, ybc, dtype, s); break;
-        case DT_F16: launch_step<_Float16>(a, ybc, dtype, s); break;
-        case DT_BF16: launch_step<bf16_t>(a, ybc, dtype, s); break;
+        case DT_F32: launch_step<float>(a, dtype, s); break;
+        case DT_F64: launch_step<double>(a, dtype, s); break;
+        case DT_F16: launch_step<_Float16>(a, dtype, s); break;
+        case DT_BF16: launch_step<bf16_t>(a, dtype, s); break;
     }
     SW_HIP(hipGetLastError());
     return SW_OK;
 }
 
-int sw_step_variants(void) { return SW_STEP_VARIANTS; }
+int sw_step_variants(void) { return NUM_TILE_VARIANTS; }
 
 int sw_set_step_variant(int dtype, int variant) {
     SW_CHECK_ARG(dtype >= 0 && dtype <= 3, "dtype must be 0..3");
-    SW_CHECK_ARG(variant >= 0 && variant < SW_STEP_VARIANTS, "unknown step variant");
+    SW_CHECK_ARG(variant >= 0 && variant < NUM_TILE_VARIANTS, "unknown step variant");
     g_step_variant[dtype] = variant;
     return SW_OK;
 }

@@ -19,8 +19,7 @@
 
 namespace sw {
 
-constexpr int ROWS = 4;            // rows (wavefronts) per workgroup
-constexpr int WG = 64 * ROWS;
+constexpr int ROWS = 4;            // rows per workgroup (a row-group)
 
 struct StepArgs {
     const void* f_in;
@@ -34,6 +33,7 @@ struct StepArgs {
     int ring;
     int nx, ny, nz, nyb;
     int x_sponge;
+    int ybc;                 // Y_PERIODIC or Y_FREESLIP
     double nu0, cs, nu_sponge;
     float scale, inv_scale;
 };
@@ -99,19 +99,57 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
     });
 }
 
+// Number of 32-bit words of chunk flags per row: bit c of a row's words is set when the
+// 64-cell chunk c holds a solid cell or a cell with a solid neighbour (x = 0 excluded).
+// The words of all rows follow the N link masks in the links buffer.
+__host__ __device__ inline uint32_t chunk_words(uint32_t nx) { return (nx + 64 * 32 - 1) / (64 * 32); }
+
+// Tuning knobs of the step kernel (the autotuner's choices):
+//   CPL : 64-cell chunks processed per iteration of a wavefront (1 or 2)
+//   PF  : issue the next chunk's loads before the current chunk's collision and stores
+//         (its loads then never wait behind the current stores in the in-order vmcnt)
+//   NT  : nontemporal (streaming) stores
+//   NTL : nontemporal loads
+//   WPB : wavefronts per workgroup (4, 8 or 16); each of the ROWS rows of a row-group is
+//         split into WPB/ROWS contiguous chunk ranges, one per wavefront
+template <int CPL_, bool PF_, bool NT_, bool NTL_ = false, int WPB_ = ROWS> struct Tile {
+    static constexpr int CPL = CPL_;
+    static constexpr bool PF = PF_;
+    static constexpr bool NT = NT_;
+    static constexpr bool NTL = NTL_;
+    static constexpr int WPB = WPB_;
+    static constexpr int NSEG = WPB_ / ROWS;
+};
+
+// The curated variant list (index = sw_set_step_variant's `variant`).
+using TileV0 = Tile<1, false, false, false, 4>;   // plain
+using TileV1 = Tile<1, false, true, false, 4>;    // nontemporal stores
+using TileV2 = Tile<1, false, true, true, 4>;     // nontemporal stores and loads
+using TileV3 = Tile<1, false, true, false, 8>;    // NT stores, rows split over 2 wavefronts
+using TileV4 = Tile<1, false, true, false, 16>;   // NT stores, rows split over 4 wavefronts
+using TileV5 = Tile<1, false, true, true, 8>;     // NT stores and loads, rows over 2 wavefronts
+using TileV6 = Tile<2, false, true, false, 4>;    // 2 chunks per iteration, NT stores
+using TileV7 = Tile<1, true, true, false, 8>;     // prefetch, NT stores, rows over 2 wavefronts
+constexpr int NUM_TILE_VARIANTS = 8;
+#define SW_TILE_LIST(X) X(0, TileV0) X(1, TileV1) X(2, TileV2) X(3, TileV3) X(4, TileV4) X(5, TileV5) \
+                        X(6, TileV6) X(7, TileV7)
+
 // One (y, z) row of one lattice; adds this lane's FP64 wall-force sums to fx_acc/fz_acc.
 // BUF selects raw-buffer addressing (all byte offsets of a population array < 4 GiB).
-template <class S, int YBC, int CPL, bool BUF>
-__device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t, int lane,
+template <class S, class TL, bool BUF>
+__device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t, int lane, int seg,
                                          double& fx_acc, double& fz_acc) {
     using C = Codec<S>;
     using T = typename C::T;
-    using B = typename C::B;
+    constexpr int CPL = TL::CPL;
+    constexpr bool PF = TL::PF && TL::CPL == 1;
     constexpr uint32_t ES = sizeof(S);
     const PopMem<S, BUF> mem(a.f_in, a.f_out);
     const uint32_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const size_t NB = (size_t)nx * ny * nz * ES;                 // bytes per population
-    const uint32_t selfrow = ((uint32_t)z * ny + y) * nx;
+    const uint32_t N = nx * ny * nz;
+    const size_t NB = (size_t)N * ES;                            // bytes per population
+    const uint32_t rowid = (uint32_t)z * ny + y;
+    const uint32_t selfrow = rowid * nx;
     const size_t selfb = (size_t)selfrow * ES;
     const T inv_scale = (T)a.inv_scale, scale = (T)a.scale;
     // (18*sqrt(2))*cs*cs evaluated left to right as lbm3d.py:146 does
@@ -125,7 +163,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
         int zs = z - CZ[i], ys = y - CY[i], j = i;
-        if constexpr (YBC == Y_PERIODIC) {
+        if (a.ybc == Y_PERIODIC) {
             ys = ys < 0 ? ys + (int)ny : (ys >= (int)ny ? ys - (int)ny : ys);      // lbm3d.py:79
         } else {
             if (ys < 0 || ys >= (int)ny) { ys = y; j = MIRY[j]; }                 // free-slip sides
@@ -135,104 +173,138 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         srow[i] = (CX[i] > 0) ? r - ES : r;
     });
     const uint32_t* lrow = a.links + selfrow;
+    const uint32_t cw = chunk_words(nx);
+    const uint32_t* rflags = a.links + N + rowid * cw;           // wave-uniform chunk flags
+    const uint32_t nch_row = (nx + 63) / 64;
+    const uint32_t seglen = (nch_row + TL::NSEG - 1) / TL::NSEG;
+    const uint32_t cbeg = min(nch_row, (uint32_t)seg * seglen);
+    const uint32_t nchunks = min(nch_row, cbeg + seglen);        // end of this wavefront's range
+    auto special_chunk = [&](uint32_t c) -> bool { return (rflags[c >> 5] >> (c & 31)) & 1u; };
 
-    for (uint32_t x0 = 0; x0 < nx; x0 += 64 * CPL) {
-        uint32_t m[CPL];
-        bool special = false;
-#pragma unroll
-        for (int c = 0; c < CPL; c++) {
-            const uint32_t x = x0 + 64 * c + lane;
-            m[c] = (x < nx) ? lrow[x] : 0u;
-            special |= (x < nx) && (x != 0) && (m[c] != 0u);
-        }
-        if (!__any(special)) {
-            // ---------------- fast path: no solid cell and no solid neighbour in the chunk(s)
-            T fl[CPL][Q];
-#pragma unroll
-            for (int c = 0; c < CPL; c++) {
-                const uint32_t x = x0 + 64 * c + lane;
-                if (x < nx && x != 0) {
-                    const uint32_t xo = x * ES;
-                    const uint32_t xpo = min(x + 1, nx - 1) * ES;
-                    static_for<0, Q>([&](auto I) {
-                        constexpr int i = I;
-                        fl[c][i] = C::decode(mem.ld(srow[i], (CX[i] < 0) ? xpo : xo), wgt<T>(i), inv_scale);
-                    });
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < CPL; c++) {
-                const uint32_t x = x0 + 64 * c + lane;
-                if (x < nx && x != 0) {
-                    const uint32_t xo = x * ES;
-                    collide<T>(fl[c], x, a, smag, [&](auto I, T v) {
-                        constexpr int i = I;
-                        mem.st((size_t)i * NB + selfb, xo, C::encode(v, wgt<T>(i), scale));
-                    });
-                }
-            }
-        } else {
-            // ---------------- general path: per-link bounce-back and wall force (:77-101)
-#pragma unroll
-            for (int c = 0; c < CPL; c++) {
-                const uint32_t x = x0 + 64 * c + lane;
-                if (!(x < nx && x != 0)) continue;
-                const uint32_t mm = m[c];
-                if (mm & SOLID_BIT) continue;                                        // :65-66
-                const uint32_t xo = x * ES;
-                const uint32_t xpo = min(x + 1, nx - 1) * ES;
-                T fl[Q];
-                T fxw = T(0), fzw = T(0);
-                bool hit = false;
-                static_for<0, Q>([&](auto I) {
-                    constexpr int i = I;
-                    constexpr T w = wgt<T>(i);
-                    if constexpr (i == 0) {
-                        fl[0] = C::decode(mem.ld(selfb, xo), w, inv_scale);
-                    } else {
-                        T val;
-                        if ((mm >> i) & 1u) {                                        // half-way bounce-back
-                            val = C::decode(mem.ld((size_t)OPP[i] * NB + selfb, xo), w, inv_scale);
-                            bool wall = true;
-                            if constexpr (GROUND_SLOT[i] >= 0)
-                                wall = !((mm >> (GROUND_BIT0 + GROUND_SLOT[i])) & 1u);
-                            if (wall) {                                              // momentum exchange
-                                const T fr2 = T(2) * (val - w);
-                                if constexpr (CX[i] > 0) fxw = fxw - fr2;
-                                if constexpr (CX[i] < 0) fxw = fxw + fr2;
-                                if constexpr (CZ[i] > 0) fzw = fzw - fr2;
-                                if constexpr (CZ[i] < 0) fzw = fzw + fr2;
-                                hit = true;
-                            }
-                        } else {
-                            val = C::decode(mem.ld(srow[i], (CX[i] < 0) ? xpo : xo), w, inv_scale);
-                        }
-                        fl[i] = val;
+    auto load_fast = [&](uint32_t x, T (&fl)[Q]) {
+        const uint32_t xo = x * ES;
+        const uint32_t xpo = min(x + 1, nx - 1) * ES;
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            fl[i] = C::decode(mem.template ld<TL::NTL>(srow[i], (CX[i] < 0) ? xpo : xo), wgt<T>(i), inv_scale);
+        });
+    };
+    auto collide_store = [&](uint32_t x, const T (&fl)[Q]) {
+        const uint32_t xo = x * ES;
+        collide<T>(fl, x, a, smag, [&](auto I, T v) {
+            constexpr int i = I;
+            mem.template st<TL::NT>((size_t)i * NB + selfb, xo, C::encode(v, wgt<T>(i), scale));
+        });
+    };
+    // general path: per-link bounce-back and wall force (:77-101)
+    auto general_cell = [&](uint32_t x) {
+        if (!(x < nx && x != 0)) return;
+        const uint32_t mm = lrow[x];
+        if (mm & SOLID_BIT) return;                                                  // :65-66
+        const uint32_t xo = x * ES;
+        const uint32_t xpo = min(x + 1, nx - 1) * ES;
+        T fl[Q];
+        T fxw = T(0), fzw = T(0);
+        bool hit = false;
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr T w = wgt<T>(i);
+            if constexpr (i == 0) {
+                fl[0] = C::decode(mem.ld(selfb, xo), w, inv_scale);
+            } else {
+                T val;
+                if ((mm >> i) & 1u) {                                                // half-way bounce-back
+                    val = C::decode(mem.ld((size_t)OPP[i] * NB + selfb, xo), w, inv_scale);
+                    bool wall = true;
+                    if constexpr (GROUND_SLOT[i] >= 0) wall = !((mm >> (GROUND_BIT0 + GROUND_SLOT[i])) & 1u);
+                    if (wall) {                                                      // momentum exchange
+                        const T fr2 = T(2) * (val - w);
+                        if constexpr (CX[i] > 0) fxw = fxw - fr2;
+                        if constexpr (CX[i] < 0) fxw = fxw + fr2;
+                        if constexpr (CZ[i] > 0) fzw = fzw - fr2;
+                        if constexpr (CZ[i] < 0) fzw = fzw + fr2;
+                        hit = true;
                     }
-                });
-                if (hit) { fx_acc += (double)fxw; fz_acc += (double)fzw; }
-                collide<T>(fl, x, a, smag, [&](auto I, T v) {
-                    constexpr int i = I;
-                    mem.st((size_t)i * NB + selfb, xo, C::encode(v, wgt<T>(i), scale));
-                });
+                } else {
+                    val = C::decode(mem.ld(srow[i], (CX[i] < 0) ? xpo : xo), w, inv_scale);
+                }
+                fl[i] = val;
+            }
+        });
+        if (hit) { fx_acc += (double)fxw; fz_acc += (double)fzw; }
+        collide_store(x, fl);
+    };
+
+    if constexpr (PF) {
+        // software-pipelined sweep: chunk c+1's loads are in flight during chunk c's work
+        T cur[Q], nxt[Q];
+        bool cur_loaded = false;
+        if (cbeg < nchunks) {
+            const uint32_t x = cbeg * 64 + lane;
+            if (!special_chunk(cbeg)) {
+                if (x < nx && x != 0) load_fast(x, cur);
+                cur_loaded = true;
+            }
+        }
+        for (uint32_t c = cbeg; c < nchunks; c++) {
+            const uint32_t x = c * 64 + lane;
+            const bool next_fast = (c + 1 < nchunks) && !special_chunk(c + 1);
+            if (cur_loaded) {
+                const uint32_t xn = x + 64;
+                if (next_fast && xn < nx) load_fast(xn, nxt);
+                if (x < nx && x != 0) collide_store(x, cur);
+            } else {
+                general_cell(x);
+                if (next_fast && x + 64 < nx) load_fast(x + 64, nxt);
+            }
+            if (next_fast) {
+                static_for<0, Q>([&](auto I) { cur[I] = nxt[I]; });
+            }
+            cur_loaded = next_fast;
+        }
+    } else {
+        for (uint32_t c0 = cbeg; c0 < nchunks; c0 += CPL) {
+            bool special = false;
+#pragma unroll
+            for (int k = 0; k < CPL; k++)
+                if (c0 + k < nchunks) special |= special_chunk(c0 + k);
+            if (!special) {
+                // ---------------- fast path: no solid cell and no solid neighbour in the chunk(s)
+                T fl[CPL][Q];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    const uint32_t x = (c0 + k) * 64 + lane;
+                    if (x < nx && x != 0) load_fast(x, fl[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    const uint32_t x = (c0 + k) * 64 + lane;
+                    if (x < nx && x != 0) collide_store(x, fl[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < CPL; k++) general_cell((c0 + k) * 64 + lane);
             }
         }
     }
     // ---------------- inlet (:67-71): x = 0 fluid cell gets feq(1, (uin[z] g, 0, 0))
-    if (lane == 0 && !(lrow[0] & SOLID_BIT)) {
+    if (seg == 0 && lane == 0 && !(lrow[0] & SOLID_BIT)) {
         T u0 = (T)a.uin[z];
         if (a.gust) u0 = u0 * (T)a.gust[t];
         const T usq15 = T(1.5) * (u0 * u0);
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            mem.st((size_t)i * NB + selfb, 0, C::encode(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale));
+            mem.template st<false>((size_t)i * NB + selfb, 0,
+                                   C::encode(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale));
         });
     }
 }
 
-// Writes this workgroup's FP64 force pair (sum over its rows) to partials[slot][z][yb].
+// Writes this workgroup's FP64 force pair (sum over its WPB wavefronts in wavefront
+// order) to partials[slot][z][yb].
+template <int WPB>
 __device__ __forceinline__ void block_force(const StepArgs& a, double fx, double fz, int z, int yb, int t) {
-    __shared__ double red[ROWS][2];
+    __shared__ double red[WPB][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     fx = wave_sum(fx);
     fz = wave_sum(fz);
@@ -240,12 +312,13 @@ __device__ __forceinline__ void block_force(const StepArgs& a, double fx, double
     __syncthreads();
     if (threadIdx.x == 0 && a.partials) {
         double sx = red[0][0], sz = red[0][1];
-        for (int r = 1; r < ROWS; r++) { sx += red[r][0]; sz += red[r][1]; }
+        for (int r = 1; r < WPB; r++) { sx += red[r][0]; sz += red[r][1]; }
         const size_t slot = (size_t)(t % a.ring);
         double* p = a.partials + ((slot * a.nz + z) * a.nyb + yb) * 2;
         p[0] = sx;
         p[1] = sz;
     }
+    __syncthreads();   // red[] is reused when a workgroup processes several row-groups
 }
 
 }  // namespace sw

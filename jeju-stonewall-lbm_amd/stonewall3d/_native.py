@@ -22,6 +22,7 @@ SIGNATURES = {
     "sw_version": (_c_int, []),
     "sw_last_error": (ctypes.c_char_p, []),
     "sw_rows_per_block": (_c_int, []),
+    "sw_links_words": (_c_ll, [_c_int, _c_int, _c_int]),
     "sw_partials_per_step": (_c_ll, [_c_int, _c_int]),
     "sw_prepare_links": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _p]),
     "sw_init": (_c_int, [_p, _c_int, _p, _p, _c_int, _c_int, _c_int, _c_flt, _c_uint, _c_flt, _p]),
@@ -32,9 +33,29 @@ SIGNATURES = {
     "sw_get_step_variant": (_c_int, [_c_int]),
     "sw_force_reduce": (_c_int, [_p, _c_int, _c_int, _p, _c_int, _c_int, _c_int, _p, _p]),
     "sw_macro": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_flt, _p, _p, _p]),
+    "sw_batch_table_bytes": (_c_ll, [_c_int]),
+    "sw_batch_upload": (_c_int, [_p, _c_int, _c_int, _p, _p]),
+    "sw_batch_download": (_c_int, [_p, _p, _c_int, _p]),
+    "sw_batch_step": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
+    "sw_batch_force_reduce": (_c_int, [_p, _c_int, _c_int, _p]),
+    "sw_batch_advance": (_c_int, [_p, _c_int, _c_int, _p]),
+    "sw_batch_graph_create": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
+    "sw_graph_launch": (_c_int, [_p, _p]),
+    "sw_graph_destroy": (_c_int, [_p]),
     "sw_to_ref_layout": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
     "sw_from_ref_layout": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
 }
+
+
+
+class SwInstance(ctypes.Structure):
+    """Mirror of struct sw_instance (include/stonewall_lbm.h)."""
+    _fields_ = [("f0", _p), ("f1", _p), ("links", _p), ("uin", _p), ("gust", _p), ("partials", _p),
+                ("force", _p), ("nu0", _c_dbl), ("cs", _c_dbl), ("nu_sponge", _c_dbl), ("scale", _c_flt),
+                ("nx", _c_int), ("ny", _c_int), ("nz", _c_int), ("ring", _c_int), ("x_sponge", _c_int),
+                ("ybc", _c_int), ("dtype", _c_int), ("t", _c_int), ("t_end", _c_int), ("parity", _c_int),
+                ("item0", _c_int), ("reserved", _c_int)]
+
 
 _lib = None
 

@@ -78,7 +78,7 @@ class Simulation:
         if uin.shape != (self.nz,):
             raise ValueError(f"uin must have shape ({self.nz},)")
         dev = self.device
-        self.links = torch.empty(self.N, dtype=torch.int32, device=dev)
+        self.links = torch.empty(nat.lib().sw_links_words(self.nx, self.ny, self.nz), dtype=torch.int32, device=dev)
         flag_d = torch.from_numpy(flag).to(dev)
         nat.call("sw_prepare_links", flag_d.data_ptr(), self.links.data_ptr(), self.nx, self.ny, self.nz, self.ybc,
                  _stream())
@@ -206,6 +206,35 @@ class Simulation:
                  self.scale, None, self.Uacc.data_ptr(), _stream())
         self.n_acc += 1
 
+    def accumulator(self):
+        """Raw Uacc (sum of sampled (rho, u)) in the reference layout, float32 (zeros if unused)."""
+        if self.Uacc is None:
+            return np.zeros((4, self.nx, self.ny, self.nz), np.float32)
+        return self._to_ref(self.Uacc, 4, 4).cpu().numpy().reshape(4, self.nx, self.ny, self.nz)
+
+    def set_accumulator(self, acc, n_acc):
+        """Restore Uacc from a reference-layout (4, nx, ny, nz) float32 array."""
+        acc = np.ascontiguousarray(acc, np.float32).reshape(-1)
+        if acc.size != 4 * self.N:
+            raise ValueError("accumulator has the wrong size")
+        if self.Uacc is None:
+            self.Uacc = torch.zeros(4 * self.N, dtype=torch.float32, device=self.device)
+        src = torch.from_numpy(acc).to(self.device)
+        nat.call("sw_from_ref_layout", src.data_ptr(), self.Uacc.data_ptr(), 4, 4, self.nx, self.ny, self.nz,
+                 _stream())
+        torch.cuda.current_stream().synchronize()
+        self.n_acc = int(n_acc)
+
+    def set_force_rows(self, rows):
+        """Restore the force record of steps 0..len(rows)-1 (resume)."""
+        if not self.record_force:
+            return
+        rows = np.asarray(rows, np.float64)
+        self.force.zero_()
+        if rows.shape[0]:
+            self.force[:rows.shape[0]] = torch.from_numpy(rows).to(self.device)
+        self._pending = 0
+
     def mean_field(self):
         """Time-mean (rho, u) = Uacc / n_acc in the reference layout (lbm3d.py:402)."""
         if self.Uacc is None:
@@ -220,10 +249,24 @@ class Simulation:
             v = w[:, None] + f.float() * np.float32(1.0 / self.scale)
         else:
             v = f
-        fluid = (self.links & (1 << 31)) == 0
+        fluid = (self.links[:self.N] & (1 << 31)) == 0
         v = torch.where(fluid[None, :], v.double(), torch.zeros((), dtype=torch.float64, device=self.device))
         idx = torch.arange(1, Q + 1, dtype=torch.float64, device=self.device)[:, None]
         return float(v.sum()), float((v * idx).sum())
+
+
+    def slices(self, y_mid, z_half):
+        """(ux, uy, uz) on the plane y = y_mid (3, nx, nz) and z = z_half (3, nx, ny) as float16,
+        from the current state (k_macro + k_slice_xz / k_slice_xy, lbm3d.py:367, :372-375).
+        The slices are cut on the device; only the planes are copied to the host."""
+        if self.U is None:
+            self.U = torch.empty(4 * self.N, dtype=torch.float32, device=self.device)
+        nat.call("sw_macro", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.nx, self.ny, self.nz,
+                 self.scale, self.U.data_ptr(), None, _stream())
+        U = self.U.view(4, self.nz, self.ny, self.nx)
+        xz = U[1:, :, y_mid, :].transpose(1, 2).to(torch.float16).cpu().numpy()
+        xy = U[1:, z_half, :, :].transpose(1, 2).to(torch.float16).cpu().numpy()
+        return xz, xy
 
 
 def bytes_per_update(dtype):
@@ -232,3 +275,158 @@ def bytes_per_update(dtype):
 
 
 __all__ = ["Simulation", "bytes_per_update", "FLUID", "WALL", "GROUND"]
+
+
+# ---------------------------------------------------------------------------------- CLI
+def _parser():
+    import argparse
+    ap = argparse.ArgumentParser(description="D3Q19 regularized-BGK + Smagorinsky LES of wind over a dry-stone "
+                                             "wall on one MI355X (flags as stonewall3d/lbm3d.py:242-264)")
+    ap.add_argument("--case", required=True, help="output name, e.g. solid / p20 / p40")
+    ap.add_argument("--porosity", type=float, required=True, help="frontal open-area fraction (0 = mortared)")
+    ap.add_argument("--H", type=int, default=40, help="wall height in cells")
+    ap.add_argument("--device", default="cuda", help="cuda / cuda:0")
+    ap.add_argument("--seed", type=int, default=7, help="stone layout seed")
+    ap.add_argument("--Re", type=float, default=15000)
+    ap.add_argument("--U0", type=float, default=0.08)
+    ap.add_argument("--Cs", type=float, default=0.16)
+    ap.add_argument("--up", type=float, default=4)
+    ap.add_argument("--down", type=float, default=16)
+    ap.add_argument("--span", type=float, default=4)
+    ap.add_argument("--height", type=float, default=8)
+    ap.add_argument("--thick", type=float, default=0.3)
+    ap.add_argument("--t-spinup", type=float, default=20)
+    ap.add_argument("--t-avg", type=float, default=30)
+    ap.add_argument("--snap-dt", type=float, default=0.25)
+    ap.add_argument("--snap3d-every", type=int, default=0,
+                    help="3D Q-criterion frames are not produced by this build; must stay 0")
+    ap.add_argument("--snap3d-ds", type=int, default=0)
+    ap.add_argument("--out", default="results")
+    ap.add_argument("--ckpt-every", type=int, default=5000)
+    ap.add_argument("--no-resume", action="store_true")
+    # additions
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"], help="population storage")
+    ap.add_argument("--ybc", default="periodic", choices=["periodic", "freeslip"], help="side boundaries")
+    ap.add_argument("--gust-seed", type=int, default=None, help="gusting inlet from this seed (default: steady)")
+    ap.add_argument("--noise", type=float, default=0.02, help="init velocity noise amplitude (lbm3d.py:340)")
+    ap.add_argument("--max-steps", type=int, default=None, help="stop after this many steps (smoke runs)")
+    return ap
+
+
+def main(argv=None):
+    """The reference's run (lbm3d.py:241-428) over the HIP kernels: same flags, same
+    checkpoint directory and the same .npz keys."""
+    import json
+    import os
+    import shutil
+    import time
+
+    from . import checkpoint
+    from .geometry import build_wall
+    from .inflow import gust_table, make_gust, reference_setup
+
+    a = _parser().parse_args(argv)
+    if a.snap3d_every:
+        raise SystemExit("--snap3d-every: 3D vortex frames are outside this build; use 0")
+    H = a.H
+    su = reference_setup(H=H, Re=a.Re, U0=a.U0, up=a.up, down=a.down, span=a.span, height=a.height, thick=a.thick,
+                         t_spinup=a.t_spinup, t_avg=a.t_avg, snap_dt=a.snap_dt)
+    nx, ny, nz, X0, T = su.nx, su.ny, su.nz, su.X0, su.T
+    n_total, n_spin, snap_every = su.n_total, su.n_spin, su.snap_every
+    if a.max_steps is not None:
+        n_total = min(n_total, a.max_steps)
+    bpu = BYTES_PER_UPDATE[nat.DTYPES[a.dtype]]
+    print(f"[grid] {nx}x{ny}x{nz} = {nx*ny*nz/1e6:.1f} M cells, ~{2*nx*ny*nz*bpu/2/1e9:.1f} GB populations | "
+          f"U_H={su.UH:.4f} nu={su.nu0:.2e} tau0={3*su.nu0+0.5:.5f}  Re={a.Re:.0f}  steps={n_total} "
+          f"(H/U = {su.t_unit:.0f} steps) dtype={a.dtype}", flush=True)
+    wall, phi = build_wall(a.porosity, H, nx, ny, nz, X0, T, seed=a.seed)
+    gust = gust_table(make_gust(a.gust_seed), n_total) if a.gust_seed is not None else None
+    sim = Simulation(wall, su.uin, nu0=su.nu0, cs=a.Cs, x_sponge=su.x_sponge, nu_sponge=0.05, ybc=a.ybc,
+                     dtype=a.dtype, gust=gust, n_steps=n_total, device=a.device)
+    y_mid, z_half = ny // 2, int(round(0.5 * H)) + 1
+    os.makedirs(a.out, exist_ok=True)
+    ckpt = os.path.join(a.out, f"{a.case}.ckpt")
+    sig = dict(grid=[nx, ny, nz], H=H, porosity=a.porosity, seed=a.seed, Re=a.Re, U0=a.U0, Cs=a.Cs, n_spin=n_spin,
+               n_total=n_total, snap_every=snap_every, snap3d=[a.snap3d_every, a.snap3d_ds], dtype=a.dtype,
+               ybc=a.ybc, gust_seed=a.gust_seed)
+    snaps_xz, snaps_xy, t_start = [], [], 0
+    if os.path.isdir(ckpt) and not a.no_resume:
+        t_start, n_acc, snaps = checkpoint.load(sim, ckpt, sig)
+        snaps_xz, snaps_xy = list(snaps["snaps_xz"]), list(snaps["snaps_xy"])
+        print(f"[{a.case}] resumed from step {t_start} ({t_start/su.t_unit:.1f} H/U)", flush=True)
+    else:
+        sim.init(noise=a.noise, seed=a.seed)
+
+    def save_ckpt(t_next):
+        checkpoint.save(sim, ckpt, t_next, sim.n_acc, sig,
+                        dict(snaps_xz=np.array(snaps_xz, np.float16).reshape(-1, 3, nx, nz),
+                             snaps_xy=np.array(snaps_xy, np.float16).reshape(-1, 3, nx, ny),
+                             snaps_q=np.zeros((0,), np.uint8)))
+        print(f"[{a.case}] checkpoint saved at step {t_next}", flush=True)
+
+    t0 = time.time()
+    for t in range(t_start, n_total):
+        sim.step(1)
+        if t >= n_spin:
+            k = t - n_spin
+            if k % 5 == 0:
+                sim.accumulate()
+            if k % snap_every == 0:
+                xz, xy = sim.slices(y_mid, z_half)
+                snaps_xz.append(xz)
+                snaps_xy.append(xy)
+        if a.ckpt_every > 0 and (t + 1) % a.ckpt_every == 0 and t + 1 < n_total:
+            save_ckpt(t + 1)
+        if t % 1000 == 0 or t == n_total - 1:
+            torch.cuda.synchronize()
+            el = time.time() - t0
+            done = t + 1 - t_start
+            mlups = done * nx * ny * nz / max(el, 1e-9) / 1e6
+            chk = ""
+            if t % 5000 == 0:
+                un = sim.macro()
+                if not np.isfinite(un).all():
+                    raise SystemExit(f"[{a.case}] diverged at step {t}")
+                chk = f" rho[{un[0].min():.3f},{un[0].max():.3f}] |u|max/U_H={np.abs(un[1:]).max()/su.UH:.2f}"
+            print(f"[{a.case}] step {t}/{n_total}  t={t/su.t_unit:.1f} H/U  {mlups:.0f} MLUPS  "
+                  f"eta {el/done*(n_total-t-1)/60:.1f} min{chk}", flush=True)
+
+    fh = sim.force_rows()
+    qref = 0.5 * su.UH ** 2
+    Fx, Fz = fh[:, :, 0].sum(1), fh[:, :, 1].sum(1)
+    M = (fh[:, :, 0] * su.zeta[None, :]).sum(1)
+    A = H * ny
+    mean = sim.mean_field()
+    if mean is None:
+        mean = np.zeros((4, nx, ny, nz), np.float32)
+    np.savez_compressed(
+        os.path.join(a.out, f"{a.case}.npz"),
+        Cd=(Fx / (qref * A)).astype(np.float32), Cl=(Fz / (qref * A)).astype(np.float32),
+        Cm=(M / (qref * A * H)).astype(np.float32),
+        fx_profile=(fh[n_spin:, :, 0].mean(0) / (qref * ny)).astype(np.float32) if n_total > n_spin
+        else np.zeros(nz, np.float32),
+        mean_xz_mid=mean[:, :, y_mid, :].astype(np.float32),
+        mean_xz_spanavg=np.where(wall.any(1)[None], np.nan, mean.mean(2)).astype(np.float32),
+        mean_xy_half=mean[:, :, :, z_half].astype(np.float32),
+        mean3d_ds2=mean[1:, ::2, ::2, ::2].astype(np.float16),
+        snaps_xz=np.array(snaps_xz, np.float16).reshape(-1, 3, nx, nz),
+        snaps_xy=np.array(snaps_xy, np.float16).reshape(-1, 3, nx, ny),
+        wall_xz=wall[:, y_mid, :], wall_xy=wall[:, :, z_half], wall_front=wall.any(0), wall_top=wall.any(2),
+        wall_ds2=wall[::2, ::2, ::2], snaps_q=np.zeros((0, 0, 0, 0), np.uint8),
+        meta=json.dumps(dict(case=a.case, porosity_target=a.porosity, porosity=float(phi), H=H, T=T, X0=X0,
+                             nx=nx, ny=ny, nz=nz, UH=su.UH, U0=a.U0, Re=a.Re, nu=su.nu0, Cs=a.Cs, n_spin=n_spin,
+                             n_avg=n_total - n_spin, steps_per_unit=su.t_unit, snap_every=snap_every, y_mid=y_mid,
+                             z_half=z_half, seed=a.seed, device=a.device, dtype=a.dtype, ybc=a.ybc,
+                             gust_seed=a.gust_seed, wall_seconds=time.time() - t0)))
+    s = slice(n_spin, None)
+    if n_total > n_spin:
+        print(f"[{a.case}] done. porosity={phi:.3f}  Cd mean={(Fx[s]/(qref*A)).mean():.3f} "
+              f"max={(Fx[s]/(qref*A)).max():.3f}  Cm mean={(M[s]/(qref*A*H)).mean():.3f}", flush=True)
+    else:
+        print(f"[{a.case}] done (stopped before the averaging window). porosity={phi:.3f}", flush=True)
+    if os.path.isdir(ckpt):
+        shutil.rmtree(ckpt)
+
+
+if __name__ == "__main__":
+    main()

@@ -6,6 +6,8 @@ orders) within 1e-12 relative; FP64 within 1e-14.  The workload-level bar (popul
 within 1e-5 relative after 1000 steps, mean Cd within 0.5 %) is asserted on the 160x80x40
 box-wall lattice in both y-boundary modes.
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -179,3 +181,68 @@ def test_headline_lattice_first_steps_bit_identical(gpu):
     assert np.array_equal(Fg, sims[1].force_rows())
     del sims
     torch.cuda.empty_cache()
+
+
+def test_resume_is_bitwise_equal_to_uninterrupted(gpu, tmp_path):
+    """Run 30 steps straight vs 12 steps, checkpoint, a fresh Simulation resumes and runs
+    18 more: populations, force rows and the accumulator agree bit for bit (the
+    reference's claim, stonewall3d/README.md:44)."""
+    from stonewall3d import checkpoint
+    flag, uin = small_case()
+    kw = dict(nu0=2e-4, x_sponge=38, ybc="periodic", n_steps=30, ring=5, gust=gust_table(make_gust(3), 30))
+    sig = dict(case="resume")
+    a = Simulation(flag, uin, **kw).init(noise=0.02, seed=2)
+    for t in range(30):
+        a.step(1)
+        if t % 5 == 0:
+            a.accumulate()
+    b = Simulation(flag, uin, **kw).init(noise=0.02, seed=2)
+    for t in range(12):
+        b.step(1)
+        if t % 5 == 0:
+            b.accumulate()
+    p = str(tmp_path / "r.ckpt")
+    checkpoint.save(b, p, 12, b.n_acc, sig)
+    del b
+    c = Simulation(flag, uin, **kw)
+    t0, n_acc, _ = checkpoint.load(c, p, sig)
+    assert t0 == 12
+    for t in range(12, 30):
+        c.step(1)
+        if t % 5 == 0:
+            c.accumulate()
+    assert np.array_equal(a.populations(), c.populations())
+    assert np.array_equal(a.force_rows(), c.force_rows())
+    assert np.array_equal(a.accumulator(), c.accumulator()) and a.n_acc == c.n_acc
+
+
+def test_cli_short_run(gpu, tmp_path):
+    """lbm3d.main with the reference's flags: 60 steps at H=8, checkpoint, resume, .npz keys."""
+    from stonewall3d import lbm3d
+    out = str(tmp_path)
+    args = ["--case", "q", "--porosity", "0.2", "--H", "8", "--t-spinup", "0.3", "--t-avg", "0.5",
+            "--out", out, "--ckpt-every", "25", "--max-steps", "60"]
+    lbm3d.main(args)
+    d = np.load(os.path.join(out, "q.npz"))
+    for k in ("Cd", "Cl", "Cm", "fx_profile", "mean_xz_mid", "mean_xz_spanavg", "mean_xy_half", "mean3d_ds2",
+              "snaps_xz", "snaps_xy", "wall_xz", "wall_xy", "wall_front", "wall_top", "wall_ds2", "snaps_q", "meta"):
+        assert k in d.files, k
+    assert d["Cd"].shape == (60,) and np.isfinite(d["Cd"]).all()
+    assert not os.path.isdir(os.path.join(out, "q.ckpt"))
+
+
+@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_every_tile_variant_bit_identical(gpu, variant, storage):
+    """Every kernel variant the autotuner may pick is bit-identical to the oracle."""
+    from stonewall3d import _native as nat
+    old = nat.lib().sw_get_step_variant(nat.DTYPES[storage])
+    nat.call("sw_set_step_variant", nat.DTYPES[storage], variant)
+    try:
+        flag, uin = small_case(nx=200, ny=9, nz=14)
+        sim, f, F = run_both(flag, uin, 12, storage, 1, gust=gust_table(make_gust(7), 12))
+        g = sim.populations()
+        assert not np.where(fluid_mask(flag), g != f, False).any()
+        assert np.allclose(sim.force_rows(), F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
+    finally:
+        nat.call("sw_set_step_variant", nat.DTYPES[storage], old)
