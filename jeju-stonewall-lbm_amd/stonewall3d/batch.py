@@ -1,0 +1,233 @@
+"""Batched form: many independent lattices advanced together on one GPU.
+
+The reference runs one lattice per process (stonewall3d/run_all.sh:12-18).  Here every
+lattice of a batch lives in HBM (two population buffers, link masks, inlet, gust table,
+force-partials ring, force record) and is described by one entry of a device-resident
+instance table (struct sw_instance).  One persistent kernel per step strides over the
+row-groups of all lattices; K steps plus the force reduction and the step advance are
+captured once as a hipGraph and replayed.  The composition of the batch lives only in
+the table, so lattices can join and leave between replays without recapture.
+
+Each lattice's arithmetic is that of the single-lattice step, so its populations and
+force rows are bit-identical alone and in any mix (given the same tile variant).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .lattice import Q, flag_from_mask
+from .lbm3d import DEFAULT_SCALE, TORCH_STORAGE, _stream
+
+
+class Lattice:
+    """Device resources of one lattice of a batch (the per-instance part of Simulation)."""
+
+    def __init__(self, flag, uin, *, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, ybc="freeslip", dtype="f32",
+                 scale=None, gust=None, n_steps=1000, ring=32, device="cuda", tag=None):
+        lib = nat.lib()
+        flag = np.asarray(flag)
+        if flag.dtype == bool:
+            flag = flag_from_mask(flag)
+        flag = np.ascontiguousarray(flag, np.uint8)
+        self.nx, self.ny, self.nz = flag.shape
+        self.N = self.nx * self.ny * self.nz
+        self.dtype = nat.DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
+        self.scale = float(DEFAULT_SCALE[self.dtype] if scale is None else scale)
+        self.ybc = {"periodic": nat.SW_Y_PERIODIC, "freeslip": nat.SW_Y_FREESLIP}[ybc] if isinstance(ybc, str) else ybc
+        self.nu0, self.cs, self.nu_sponge = float(nu0), float(cs), float(nu_sponge)
+        self.x_sponge = self.nx if x_sponge is None else int(x_sponge)
+        self.n_steps, self.ring, self.tag = int(n_steps), int(ring), tag
+        self.device = torch.device(device)
+        dev = self.device
+        self.links = torch.empty(lib.sw_links_words(self.nx, self.ny, self.nz), dtype=torch.int32, device=dev)
+        flag_d = torch.from_numpy(flag).to(dev)
+        nat.call("sw_prepare_links", flag_d.data_ptr(), self.links.data_ptr(), self.nx, self.ny, self.nz, self.ybc,
+                 _stream())
+        del flag_d
+        st = TORCH_STORAGE[self.dtype]
+        self.f = [torch.empty(Q * self.N, dtype=st, device=dev), torch.zeros(Q * self.N, dtype=st, device=dev)]
+        self.uin = torch.from_numpy(np.ascontiguousarray(uin, np.float32)).to(dev)
+        self.gust = None
+        if gust is not None:
+            self.gust = torch.from_numpy(np.ascontiguousarray(gust[:self.n_steps], np.float32)).to(dev)
+        self.partials = torch.zeros(self.ring * lib.sw_partials_per_step(self.ny, self.nz), dtype=torch.float64,
+                                    device=dev)
+        self.force = torch.zeros((self.n_steps, self.nz, 2), dtype=torch.float64, device=dev)
+        self.t = 0
+        self.parity = 0
+
+    @property
+    def nbytes(self):
+        return sum(x.numel() * x.element_size() for x in (self.links, self.f[0], self.f[1], self.uin, self.partials,
+                                                          self.force)) + \
+            (self.gust.numel() * 4 if self.gust is not None else 0)
+
+    def init(self, noise=0.0, seed=0):
+        nat.call("sw_init", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.uin.data_ptr(), self.nx,
+                 self.ny, self.nz, float(noise), int(seed) & 0xFFFFFFFF, self.scale, _stream())
+        self.t, self.parity = 0, 0
+        return self
+
+    def instance(self):
+        e = nat.SwInstance()
+        e.f0, e.f1 = self.f[0].data_ptr(), self.f[1].data_ptr()
+        e.links, e.uin = self.links.data_ptr(), self.uin.data_ptr()
+        e.gust = self.gust.data_ptr() if self.gust is not None else None
+        e.partials, e.force = self.partials.data_ptr(), self.force.data_ptr()
+        e.nu0, e.cs, e.nu_sponge, e.scale = self.nu0, self.cs, self.nu_sponge, self.scale
+        e.nx, e.ny, e.nz = self.nx, self.ny, self.nz
+        e.ring, e.x_sponge, e.ybc, e.dtype = self.ring, self.x_sponge, self.ybc, self.dtype
+        e.t, e.t_end, e.parity = self.t, self.n_steps, self.parity
+        return e
+
+    @property
+    def done(self):
+        return self.t >= self.n_steps
+
+    def current(self):
+        """Population buffer holding step t."""
+        return self.f[self.parity]
+
+    def populations(self):
+        """Raw stored populations of step t in the reference layout (19, nx, ny, nz)."""
+        src = self.current()
+        out = torch.empty_like(src)
+        nat.call("sw_to_ref_layout", src.data_ptr(), out.data_ptr(), nat.DTYPE_BYTES[self.dtype], Q, self.nx, self.ny,
+                 self.nz, _stream())
+        a = out.cpu()
+        if self.dtype in (nat.SW_F16, nat.SW_BF16):
+            a = a.view(torch.int16).numpy().view(np.uint16)
+        else:
+            a = a.numpy()
+        return a.reshape(Q, self.nx, self.ny, self.nz)
+
+    def force_rows(self):
+        return self.force[:self.t].cpu().numpy()
+
+
+def default_blocks(device=None, per_cu=2):
+    props = torch.cuda.get_device_properties(device or torch.cuda.current_device())
+    return int(props.multi_processor_count) * per_cu
+
+
+class Batch:
+    """A set of lattices of one dtype advanced K steps per launch sequence.
+
+    K        : steps per chunk (one graph replay); every lattice's ring must be >= K.
+    blocks   : persistent workgroups per step launch (default: 2 per compute unit).
+    variant  : tile variant (sw_set_step_variant numbering); default: the library's
+               current choice for this dtype.
+    graph    : capture the chunk as a hipGraph (else launch it eagerly).
+    """
+
+    def __init__(self, dtype="f32", max_instances=64, K=16, blocks=None, variant=None, graph=True, device="cuda"):
+        lib = nat.lib()
+        self.dtype = nat.DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
+        self.max_instances, self.K = int(max_instances), int(K)
+        self.device = torch.device(device)
+        self.blocks = int(blocks) if blocks else default_blocks(self.device)
+        self.variant = lib.sw_get_step_variant(self.dtype) if variant is None else int(variant)
+        self.table = torch.zeros(lib.sw_batch_table_bytes(self.max_instances), dtype=torch.uint8, device=self.device)
+        self.members = []
+        self.big = False
+        self.use_graph = bool(graph)
+        self._graph = None
+        self._graph_key = None
+        self._upload()
+
+    # ---------------------------------------------------------------- composition
+    def _upload(self):
+        n = len(self.members)
+        arr = (nat.SwInstance * max(n, 1))()
+        for i, m in enumerate(self.members):
+            arr[i] = m.instance()
+        nat.call("sw_batch_upload", ctypes.cast(arr, ctypes.c_void_p), n, self.max_instances, self.table.data_ptr(),
+                 _stream())
+        esize = nat.DTYPE_BYTES[self.dtype]
+        self.big = any(Q * m.N * esize >= (1 << 32) for m in self.members)
+
+    def sync(self):
+        """Pull t / parity of every member back from the device table."""
+        n = len(self.members)
+        if n == 0:
+            return
+        arr = (nat.SwInstance * n)()
+        nat.call("sw_batch_download", self.table.data_ptr(), ctypes.cast(arr, ctypes.c_void_p), n, _stream())
+        for m, e in zip(self.members, arr):
+            m.t, m.parity = int(e.t), int(e.parity)
+
+    def add(self, lattice):
+        if lattice.dtype != self.dtype:
+            raise ValueError("a batch holds lattices of one dtype")
+        if lattice.ring < self.K:
+            raise ValueError(f"lattice ring {lattice.ring} < K={self.K}")
+        if len(self.members) >= self.max_instances:
+            raise RuntimeError("batch is full")
+        self.sync()
+        self.members.append(lattice)
+        self._upload()
+
+    def remove(self, lattice):
+        self.sync()
+        self.members = [m for m in self.members if m is not lattice]
+        self._upload()
+
+    def retire_finished(self):
+        """Remove and return the members that reached their end step."""
+        self.sync()
+        done = [m for m in self.members if m.done]
+        if done:
+            self.members = [m for m in self.members if not m.done]
+            self._upload()
+        return done
+
+    # ---------------------------------------------------------------- stepping
+    def _ensure_graph(self):
+        key = (self.dtype, self.K, self.blocks, self.variant, self.big)
+        if self._graph is not None and self._graph_key == key:
+            return
+        self.close()
+        h = ctypes.c_void_p()
+        nat.call("sw_batch_graph_create", self.table.data_ptr(), self.max_instances, self.dtype, self.K, self.blocks,
+                 self.variant, int(self.big), ctypes.byref(h))
+        self._graph, self._graph_key = h, key
+
+    def run_chunk(self):
+        """Advance every unfinished member by up to K steps (asynchronous on the current stream)."""
+        if not self.members:
+            return
+        s = _stream()
+        if self.use_graph:
+            self._ensure_graph()
+            nat.call("sw_graph_launch", self._graph, s)
+        else:
+            for k in range(self.K):
+                nat.call("sw_batch_step", self.table.data_ptr(), self.dtype, k, self.blocks, self.variant,
+                         int(self.big), s)
+            nat.call("sw_batch_force_reduce", self.table.data_ptr(), self.max_instances, self.K, s)
+            nat.call("sw_batch_advance", self.table.data_ptr(), self.max_instances, self.K, s)
+
+    def run(self, chunks):
+        for _ in range(int(chunks)):
+            self.run_chunk()
+        self.sync()
+
+    def run_to_completion(self):
+        while True:
+            self.sync()
+            if all(m.done for m in self.members):
+                return
+            self.run_chunk()
+
+    def close(self):
+        if self._graph is not None:
+            nat.lib().sw_graph_destroy(self._graph)
+            self._graph = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

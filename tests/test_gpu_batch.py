@@ -1,0 +1,147 @@
+"""Batched and serving forms: each lattice's results are bit-identical alone and in any
+mix, graph replays equal eager launches, lattices join mid-run (needs an MI355X)."""
+import numpy as np
+import pytest
+import torch
+
+from stonewall3d import _native as nat
+from stonewall3d import workloads
+from stonewall3d.autotune import Autotuner
+from stonewall3d.batch import Batch, Lattice
+from stonewall3d.lbm3d import Simulation
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(6, 0.2, 11, 37), (8, 0.35, 12, 40), (5, 0.0, 13, 29)]     # H, porosity, seed, steps
+VARIANT = 7
+
+
+def wl(H, por, seed, steps):
+    return workloads.porous_wall(H=H, porosity=por, seed=seed, n_steps=steps)
+
+
+def alone(w, dtype, variant, seed):
+    code = nat.DTYPES[dtype]
+    old = nat.lib().sw_get_step_variant(code)
+    nat.call("sw_set_step_variant", code, variant)
+    try:
+        sim = Simulation(w.flag, w.uin, dtype=dtype, **w.sim_kwargs()).init(noise=0.02, seed=seed)
+        sim.step(w.n_steps)
+        return sim.populations(), sim.force_rows()
+    finally:
+        nat.call("sw_set_step_variant", code, old)
+
+
+def lattice(w, dtype, seed, K):
+    return Lattice(w.flag, w.uin, dtype=dtype, ring=K, **w.sim_kwargs()).init(noise=0.02, seed=seed)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+def test_alone_equals_mix(gpu, dtype):
+    K = 8
+    ws = [wl(*c) for c in CASES]
+    ref = [alone(w, dtype, VARIANT, c[2]) for w, c in zip(ws, CASES)]
+    b = Batch(dtype, max_instances=8, K=K, variant=VARIANT, graph=True)
+    lats = [lattice(w, dtype, c[2], K) for w, c in zip(ws, CASES)]
+    for lat in lats:
+        b.add(lat)
+    b.run_to_completion()
+    for lat, (p, F) in zip(lats, ref):
+        assert lat.t == lat.n_steps
+        assert np.array_equal(lat.populations(), p)
+        assert np.array_equal(lat.force_rows(), F)
+    b.close()
+
+
+def test_graph_replays_equal_eager(gpu):
+    K = 8
+    ws = [wl(*c) for c in CASES]
+    outs = []
+    for graph in (True, False):
+        b = Batch("f32", max_instances=8, K=K, variant=VARIANT, graph=graph)
+        lats = [lattice(w, "f32", c[2], K) for w, c in zip(ws, CASES)]
+        for lat in lats:
+            b.add(lat)
+        snaps = []
+        for _ in range(3):
+            b.run(1)                                       # replay, then host work in between
+            torch.cuda.synchronize()
+            snaps.append([lat.populations() for lat in lats])
+        b.run_to_completion()
+        outs.append((snaps, [lat.populations() for lat in lats], [lat.force_rows() for lat in lats]))
+        b.close()
+    (s1, p1, f1), (s2, p2, f2) = outs
+    for a, c in zip(s1, s2):
+        for x, y in zip(a, c):
+            assert np.array_equal(x, y)
+    for x, y in zip(p1 + f1, p2 + f2):
+        assert np.array_equal(x, y)
+
+
+def test_join_mid_run(gpu):
+    K = 8
+    ws = [wl(*c) for c in CASES]
+    ref = [alone(w, "f32", VARIANT, c[2]) for w, c in zip(ws, CASES)]
+    b = Batch("f32", max_instances=8, K=K, variant=VARIANT, graph=True)
+    lats = [lattice(w, "f32", c[2], K) for w, c in zip(ws, CASES)]
+    b.add(lats[0])
+    b.run(2)
+    b.add(lats[1])
+    b.run(1)
+    b.add(lats[2])
+    done = []
+    while b.members:
+        b.run_chunk()
+        done += b.retire_finished()
+    assert len(done) == 3
+    for lat, (p, F) in zip(lats, ref):
+        assert np.array_equal(lat.populations(), p)
+        assert np.array_equal(lat.force_rows(), F)
+    b.close()
+
+
+def test_server_results_alone_equal_mix(gpu):
+    from stonewall3d.serving import Request, Server
+    reqs = [Request(0, 0.0, 6, 0.2, 21, 33, "f32"), Request(1, 0.0, 5, 0.4, 22, 41, "f16"),
+            Request(2, 0.0, 7, 0.1, 23, 26, "f32"), Request(3, 0.0, 6, 0.3, 24, 50, "f16")]
+    srv = Server(budget_bytes=4e9, K=8, keep_force=True, variants={"f32": VARIANT, "f16": VARIANT})
+    results, rep = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert rep["requests"] == 4 and rep["p99_latency_s"] >= rep["p50_latency_s"]
+    for r in results:
+        req = reqs[r.rid]
+        w = workloads.porous_wall(H=req.H, porosity=req.porosity, seed=req.seed, n_steps=req.steps)
+        _, F = alone(w, req.dtype, VARIANT, req.seed)
+        assert np.array_equal(r.force, F), f"request {r.rid} differs in the mix"
+
+
+def test_server_respects_budget(gpu):
+    from stonewall3d.serving import Request, Server, lattice_bytes
+    reqs = [Request(k, 0.0, 6, 0.2, 30 + k, 24, "f32") for k in range(4)]
+    one = lattice_bytes(6, "f32", 24, 8)
+    srv = Server(budget_bytes=2.5 * one, K=8)
+    peak = []
+    orig = srv._batch
+
+    def spy(dtype):
+        b = orig(dtype)
+        peak.append(len(b.members))
+        return b
+
+    srv._batch = spy
+    results, rep = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert rep["requests"] == 4
+    assert max(peak) <= 2                 # never more than two resident at once
+
+
+def test_autotuner_selects_variant(gpu):
+    w = wl(8, 0.2, 3, 200)
+    sim = Simulation(w.flag, w.uin, **w.sim_kwargs()).init()
+    old = nat.lib().sw_get_step_variant(0)
+    try:
+        res = Autotuner().tune_step(sim, variants=[0, 1, 7], steps=5)
+        assert set(res) == {0, 1, 7}
+        assert nat.lib().sw_get_step_variant(0) == min(res, key=res.get)
+    finally:
+        nat.call("sw_set_step_variant", 0, old)
