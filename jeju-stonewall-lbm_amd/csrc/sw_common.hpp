@@ -215,6 +215,17 @@ __device__ __forceinline__ T feq(T rho, T ux, T uy, T uz, T usq15) {
     }
 }
 
+// x / 3 rounded to nearest: x*RN(1/3) corrected by one FMA residual step.  Checked
+// exhaustively against IEEE division for all 2^32 float inputs: identical for every finite
+// x except -0 (which a density never is).  The double form keeps the plain division.
+__device__ __forceinline__ float div3(float x) {
+    constexpr float C3 = 1.0f / 3.0f;
+    const float q0 = x * C3;
+    const float r = __builtin_fmaf(-q0, 3.0f, x);
+    return __builtin_fmaf(r, C3, q0);
+}
+__device__ __forceinline__ double div3(double x) { return x / 3.0; }
+
 // PCG hash (Jarzynski & Olano, JCGT 2020) for the optional init noise
 __device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
     uint32_t state = v * 747796405u + 2891336453u;

@@ -6,7 +6,7 @@ namespace sw {
 
 // ------------------------------------------------------------------ step
 template <class S, class TL, bool BUF>
-__global__ __launch_bounds__(64 * TL::WPB) void k_step(StepArgs a) {
+__global__ __launch_bounds__(64 * TL::WPB, TL::OCC) void k_step(StepArgs a) {
     // the wave index is uniform per wavefront; readfirstlane lets the compiler keep every
     // row-derived pointer in scalar registers
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

@@ -62,7 +62,7 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
         if constexpr (cy * cz != 0 && i != 15) pyz = cy * cz > 0 ? pyz + v : pyz - v;
     });
     const T ux = mx / rho, uy = my / rho, uz = mz / rho;                        // :128-138
-    const T third = rho / T(3);
+    const T third = div3(rho);
     const T nxx = (pxx - third) - (rho * ux) * ux;
     const T nyy = (pyy - third) - (rho * uy) * uy;
     const T nzz = (pzz - third) - (rho * uz) * uz;
@@ -83,11 +83,20 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
     const T usq15 = T(1.5) * ((ux * ux + uy * uy) + uz * uz);
 
     // ---- regularized collision (:149-155)
+    // Directions come in opposite pairs (i, i+1), i odd, c_{i+1} = -c_i.  Negation is exact
+    // in IEEE arithmetic, so the pair's cu values are +-cu, their (0.5*cu)*cu terms are
+    // equal and their stress projections qq are identical: each is computed once, and the
+    // results are bit-identical to evaluating lbm3d.py:153-155 separately per direction.
     constexpr T third_c = T(1) / T(3);
     constexpr T A1 = T(1) - third_c;      // (c*c - 1/3) for |c| = 1
     constexpr T A0 = T(0) - third_c;      // (c*c - 1/3) for c = 0
-    static_for<0, Q>([&](auto I) {
-        constexpr int i = I;
+    {
+        constexpr T w = wgt<T>(0);
+        const T qq = (A0 * nxx + A0 * nyy) + A0 * nzz;
+        st(std::integral_constant<int, 0>{}, feq<0, T>(rho, ux, uy, uz, usq15) + ((keep * w) * T(4.5)) * qq);
+    }
+    static_for<0, 9>([&](auto P) {
+        constexpr int i = 2 * P + 1, j = i + 1;
         constexpr int cx = CX[i], cy = CY[i], cz = CZ[i];
         constexpr T w = wgt<T>(i);
         constexpr T ax = cx ? A1 : A0, ay = cy ? A1 : A0, az = cz ? A1 : A0;
@@ -95,7 +104,12 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
         if constexpr (cx * cy != 0) qq = qq + T(2) * (cx * cy > 0 ? nxy : -nxy);
         if constexpr (cx * cz != 0) qq = qq + T(2) * (cx * cz > 0 ? nxz : -nxz);
         if constexpr (cy * cz != 0) qq = qq + T(2) * (cy * cz > 0 ? nyz : -nyz);
-        st(I, feq<i, T>(rho, ux, uy, uz, usq15) + ((keep * w) * T(4.5)) * qq);
+        const T neq = ((keep * w) * T(4.5)) * qq;
+        const T cu = T(3) * cdotu<i>(ux, uy, uz);
+        const T h = (T(0.5) * cu) * cu;
+        const T wr = w * rho;
+        st(std::integral_constant<int, i>{}, wr * (((T(1) + cu) + h) - usq15) + neq);
+        st(std::integral_constant<int, j>{}, wr * (((T(1) + (-cu)) + h) - usq15) + neq);
     });
 }
 
@@ -112,7 +126,11 @@ __host__ __device__ inline uint32_t chunk_words(uint32_t nx) { return (nx + 64 *
 //   NTL : nontemporal loads
 //   WPB : wavefronts per workgroup (4, 8 or 16); each of the ROWS rows of a row-group is
 //         split into WPB/ROWS contiguous chunk ranges, one per wavefront
-template <int CPL_, bool PF_, bool NT_, bool NTL_ = false, int WPB_ = ROWS> struct Tile {
+//   OCC : minimum wavefronts per SIMD the compiler must fit (0 = compiler's choice).  A cap
+//         of 8 forces 64 VGPRs and spills the 16-bit kernels to scratch (measured 100x
+//         slower), so the list below uses 0 throughout.
+template <int CPL_, bool PF_, bool NT_, bool NTL_ = false, int WPB_ = ROWS, int OCC_ = 0> struct Tile {
+    static constexpr int OCC = OCC_;
     static constexpr int CPL = CPL_;
     static constexpr bool PF = PF_;
     static constexpr bool NT = NT_;
@@ -246,6 +264,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                 cur_loaded = true;
             }
         }
+#pragma clang loop unroll(disable)
         for (uint32_t c = cbeg; c < nchunks; c++) {
             const uint32_t x = c * 64 + lane;
             const bool next_fast = (c + 1 < nchunks) && !special_chunk(c + 1);
@@ -263,6 +282,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             cur_loaded = next_fast;
         }
     } else {
+#pragma clang loop unroll(disable)
         for (uint32_t c0 = cbeg; c0 < nchunks; c0 += CPL) {
             bool special = false;
 #pragma unroll
