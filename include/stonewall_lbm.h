@@ -114,6 +114,12 @@ int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, 
 int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale, float* U,
              float* Uacc, void* stream);
 
+/* Max-pooled positive Q-criterion frame of U (from sw_macro) over ds^3 blocks of the crop
+ * starting at (x0, 0, 0): out[i][j][k], (qx, qy, qz).  Replaces k_qcoarse + q_at,
+ * lbm3d.py:203-238, launched at lbm3d.py:316. */
+int sw_qcoarse(const float* U, const uint32_t* links, int nx, int ny, int nz, int x0, int ds, int qx, int qy, int qz,
+               float* out, void* stream);
+
 /* ---------------------------------------------------------------- batched form
  * No reference counterpart: the reference runs one lattice per process (run_all.sh).
  * Many independent lattices of one dtype advance together: one persistent launch per

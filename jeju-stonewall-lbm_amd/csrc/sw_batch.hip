@@ -124,6 +124,8 @@ static void launch_batch_v(const void* table, int k, int blocks, hipStream_t s) 
 template <class S>
 static void launch_batch(const void* table, int k, int blocks, int variant, bool big, hipStream_t s) {
     if (big) { launch_batch_v<S, TileV0, false>(table, k, blocks, s); return; }
+    // pairs need 16-bit storage and even nx for every lattice; the host checks nx
+    if ((variant == 8 || variant == 9) && sizeof(S) != 2) variant = 1;
     switch (variant) {
 #define SW_CASE(id, TL) case id: launch_batch_v<S, TL, true>(table, k, blocks, s); break;
         SW_TILE_LIST(SW_CASE)

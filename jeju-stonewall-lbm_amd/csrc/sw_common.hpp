@@ -153,6 +153,14 @@ template <class S> struct PopMem<S, true> {
     using B = typename Codec<S>::B;
     __amdgpu_buffer_rsrc_t rin, rout;
     __device__ __forceinline__ PopMem(const void* fin, void* fout) : rin(make_rsrc(fin)), rout(make_rsrc(fout)) {}
+    // 32-bit access (two adjacent 16-bit populations)
+    __device__ __forceinline__ uint32_t ld32(size_t row_bytes, uint32_t lane_bytes) const {
+        return buf_load<uint32_t>(rin, lane_bytes, (uint32_t)row_bytes);
+    }
+    template <bool NT = false>
+    __device__ __forceinline__ void st32(size_t row_bytes, uint32_t lane_bytes, uint32_t v) const {
+        buf_store<uint32_t, NT ? 2 : 0>(v, rout, lane_bytes, (uint32_t)row_bytes);
+    }
     template <bool NT = false>
     __device__ __forceinline__ B ld(size_t row_bytes, uint32_t lane_bytes) const {
         return buf_load<B, NT ? 2 : 0>(rin, lane_bytes, (uint32_t)row_bytes);
@@ -168,6 +176,14 @@ template <class S> struct PopMem<S, false> {
     char* out;
     __device__ __forceinline__ PopMem(const void* fin, void* fout)
         : in(static_cast<const char*>(fin)), out(static_cast<char*>(fout)) {}
+    __device__ __forceinline__ uint32_t ld32(size_t row_bytes, uint32_t lane_bytes) const {
+        return *reinterpret_cast<const uint32_t*>(in + row_bytes + lane_bytes);
+    }
+    template <bool NT = false>
+    __device__ __forceinline__ void st32(size_t row_bytes, uint32_t lane_bytes, uint32_t v) const {
+        uint32_t* p = reinterpret_cast<uint32_t*>(out + row_bytes + lane_bytes);
+        if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
+    }
     template <bool NT = false>
     __device__ __forceinline__ B ld(size_t row_bytes, uint32_t lane_bytes) const {
         const B* p = reinterpret_cast<const B*>(in + row_bytes + lane_bytes);
@@ -233,6 +249,20 @@ __device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
     return (word >> 22u) ^ word;
 }
 __device__ __forceinline__ float u01(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+
+// ------------------------------------------------------------- lane shifts (whole wavefront)
+// dpp wave_shr:1 — lane L receives lane L-1's value, lane 0 receives `edge`
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t edge) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, 0x138, 0xF, 0xF, false);
+}
+// dpp wave_shl:1 — lane L receives lane L+1's value, lane 63 receives `edge`
+__device__ __forceinline__ uint32_t wave_shl1(uint32_t v, uint32_t edge) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, 0x130, 0xF, 0xF, false);
+}
+// two 16-bit halves: low = hi half of `lo`, high = lo half of `hi`
+__device__ __forceinline__ uint32_t mid16(uint32_t hi, uint32_t lo) {
+    return __builtin_amdgcn_alignbit(hi, lo, 16);
+}
 
 // ------------------------------------------------------------- wave reductions
 __device__ __forceinline__ double wave_sum(double v) {

@@ -131,6 +131,48 @@ __global__ void k_macro(const S* __restrict__ f, const uint32_t* __restrict__ li
     }
 }
 
+// ------------------------------------------------------------------ Q-criterion frames (lbm3d.py:203-238)
+// U is [4][z][y][x] (rho, ux, uy, uz) as sw_macro writes it; a cell is solid when its
+// link mask has SOLID_BIT.  One thread per output voxel (i, j, k) of the (qx, qy, qz)
+// frame; Q = -1/2 du_i/dx_j du_j/dx_i by central differences, 0 next to solids, max-pooled
+// (positive part) over the ds^3 block at (x0 + i*ds, j*ds, k*ds).  Operand order as q_at.
+__global__ void k_qcoarse(const float* __restrict__ U, const uint32_t* __restrict__ links, int nx, int ny, int nz,
+                          int x0, int ds, int qx, int qy, int qz, float* __restrict__ out) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= qx * qy * qz) return;
+    const int k = gid % qz, j = (gid / qz) % qy, i = gid / (qz * qy);
+    const size_t N = (size_t)nx * ny * nz;
+    auto C = [&](int x, int y, int z) { return ((size_t)z * ny + y) * nx + x; };
+    auto solid = [&](int x, int y, int z) { return (links[C(x, y, z)] & SOLID_BIT) != 0u; };
+    const float* u1 = U + N;
+    const float* u2 = U + 2 * N;
+    const float* u3 = U + 3 * N;
+    float best = 0.0f;
+    for (int a = 0; a < ds; a++)
+        for (int b = 0; b < ds; b++)
+            for (int c = 0; c < ds; c++) {
+                const int x = x0 + i * ds + a, y = j * ds + b, z = k * ds + c;
+                if (!(x >= 1 && x < nx - 1 && z >= 1 && z < nz - 1)) continue;
+                const int yp = (y + 1) % ny, ym = (y - 1 + ny) % ny;
+                float q = 0.0f;
+                if (!(solid(x, y, z) || solid(x + 1, y, z) || solid(x - 1, y, z) || solid(x, yp, z) ||
+                      solid(x, ym, z) || solid(x, y, z + 1) || solid(x, y, z - 1))) {
+                    const float a11 = 0.5f * (u1[C(x + 1, y, z)] - u1[C(x - 1, y, z)]);
+                    const float a12 = 0.5f * (u1[C(x, yp, z)] - u1[C(x, ym, z)]);
+                    const float a13 = 0.5f * (u1[C(x, y, z + 1)] - u1[C(x, y, z - 1)]);
+                    const float a21 = 0.5f * (u2[C(x + 1, y, z)] - u2[C(x - 1, y, z)]);
+                    const float a22 = 0.5f * (u2[C(x, yp, z)] - u2[C(x, ym, z)]);
+                    const float a23 = 0.5f * (u2[C(x, y, z + 1)] - u2[C(x, y, z - 1)]);
+                    const float a31 = 0.5f * (u3[C(x + 1, y, z)] - u3[C(x - 1, y, z)]);
+                    const float a32 = 0.5f * (u3[C(x, yp, z)] - u3[C(x, ym, z)]);
+                    const float a33 = 0.5f * (u3[C(x, y, z + 1)] - u3[C(x, y, z - 1)]);
+                    q = -0.5f * (((a11 * a11 + a22 * a22) + a33 * a33) + 2.0f * ((a12 * a21 + a13 * a31) + a23 * a32));
+                }
+                best = q > best ? q : best;
+            }
+    out[((size_t)i * qy + j) * qz + k] = best;
+}
+
 // ------------------------------------------------------------------ force rows
 // force[t][z][c] = sum over yb of partials[t % ring][z][yb][c], in a fixed order:
 // lane-strided sums, then a fixed butterfly; lane 0's value is written.
@@ -207,7 +249,9 @@ static bool fits_buffer(const StepArgs& a, size_t elem) {
 template <class S>
 static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
     if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV0, false>(a, s); return; }
-    switch (g_step_variant[dtype]) {
+    int v = g_step_variant[dtype];
+    if ((v == 8 || v == 9) && (sizeof(S) != 2 || (a.nx & 1))) v = 1;     // pairs: 16-bit storage, even nx
+    switch (v) {
 #define SW_CASE(id, TL) case id: launch_step_v<S, TL, true>(a, s); break;
         SW_TILE_LIST(SW_CASE)
 #undef SW_CASE
@@ -341,6 +385,21 @@ int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, in
         case DT_F16: hipLaunchKernelGGL(k_macro<_Float16>, grid, blk, 0, s, (const _Float16*)f, links, nx, ny, nz, inv, U, Uacc); break;
         case DT_BF16: hipLaunchKernelGGL(k_macro<bf16_t>, grid, blk, 0, s, (const bf16_t*)f, links, nx, ny, nz, inv, U, Uacc); break;
     }
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_qcoarse(const float* U, const uint32_t* links, int nx, int ny, int nz, int x0, int ds, int qx, int qy, int qz,
+               float* out, void* stream) {
+    SW_CHECK_ARG(U && links && out, "U, links and out must be non-null");
+    SW_CHECK_RC(check_dims(nx, ny, nz));
+    SW_CHECK_ARG(ds >= 1 && qx >= 1 && qy >= 1 && qz >= 1 && x0 >= 0, "need ds, qx, qy, qz >= 1 and x0 >= 0");
+    SW_CHECK_ARG(x0 + (long long)qx * ds <= nx && (long long)qy * ds <= ny && (long long)qz * ds <= nz,
+                 "the pooled crop must lie inside the lattice");
+    const long long n = (long long)qx * qy * qz;
+    SW_CHECK_ARG(n < (1LL << 31), "frame too large");
+    hipLaunchKernelGGL(k_qcoarse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, U, links, nx,
+                       ny, nz, x0, ds, qx, qy, qz, out);
     SW_HIP(hipGetLastError());
     return SW_OK;
 }

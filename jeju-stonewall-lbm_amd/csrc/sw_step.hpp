@@ -129,7 +129,12 @@ __host__ __device__ inline uint32_t chunk_words(uint32_t nx) { return (nx + 64 *
 //   OCC : minimum wavefronts per SIMD the compiler must fit (0 = compiler's choice).  A cap
 //         of 8 forces 64 VGPRs and spills the 16-bit kernels to scratch (measured 100x
 //         slower), so the list below uses 0 throughout.
-template <int CPL_, bool PF_, bool NT_, bool NTL_ = false, int WPB_ = ROWS, int OCC_ = 0> struct Tile {
+//   PAIR: 16-bit storage only — a lane owns two adjacent cells, each link is one aligned
+//         32-bit load (x-1 / x+1 neighbours by whole-wavefront DPP shifts) and one 32-bit
+//         store; other storage forms ignore it
+template <int CPL_, bool PF_, bool NT_, bool NTL_ = false, int WPB_ = ROWS, int OCC_ = 0, bool PAIR_ = false>
+struct Tile {
+    static constexpr bool PAIR = PAIR_;
     static constexpr int OCC = OCC_;
     static constexpr int CPL = CPL_;
     static constexpr bool PF = PF_;
@@ -148,9 +153,11 @@ using TileV4 = Tile<1, false, true, false, 16>;   // NT stores, rows split over 
 using TileV5 = Tile<1, false, true, true, 8>;     // NT stores and loads, rows over 2 wavefronts
 using TileV6 = Tile<2, false, true, false, 4>;    // 2 chunks per iteration, NT stores
 using TileV7 = Tile<1, true, true, false, 8>;     // prefetch, NT stores, rows over 2 wavefronts
-constexpr int NUM_TILE_VARIANTS = 8;
+using TileV8 = Tile<1, false, true, false, 4, 0, true>;   // 16-bit pairs, NT stores
+using TileV9 = Tile<1, false, true, false, 8, 0, true>;   // 16-bit pairs, NT stores, rows over 2 wavefronts
+constexpr int NUM_TILE_VARIANTS = 10;
 #define SW_TILE_LIST(X) X(0, TileV0) X(1, TileV1) X(2, TileV2) X(3, TileV3) X(4, TileV4) X(5, TileV5) \
-                        X(6, TileV6) X(7, TileV7)
+                        X(6, TileV6) X(7, TileV7) X(8, TileV8) X(9, TileV9)
 
 // One (y, z) row of one lattice; adds this lane's FP64 wall-force sums to fx_acc/fz_acc.
 // BUF selects raw-buffer addressing (all byte offsets of a population array < 4 GiB).
@@ -253,7 +260,81 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         collide_store(x, fl);
     };
 
-    if constexpr (PF) {
+    bool inlet_done = false;
+    if constexpr (TL::PAIR && ES == 2) {
+        // 128-cell chunks; lane L owns cells x0 + 2L and x0 + 2L + 1 (nx is even)
+        const uint32_t npc = (nx + 127) / 128;
+        const uint32_t plen = (npc + TL::NSEG - 1) / TL::NSEG;
+        const uint32_t pbeg = min(npc, (uint32_t)seg * plen);
+        const uint32_t pend = min(npc, pbeg + plen);
+#pragma clang loop unroll(disable)
+        for (uint32_t c = pbeg; c < pend; c++) {
+            const uint32_t x0 = c * 128;
+            const bool special = special_chunk(2 * c) || (2 * c + 1 < nch_row && special_chunk(2 * c + 1));
+            if (special) {
+                general_cell(x0 + lane);
+                general_cell(x0 + 64 + lane);
+                continue;
+            }
+            const uint32_t xa = x0 + 2 * lane;
+            const uint32_t xl = min(xa, nx - 2);             // lanes past the row end read in-row
+            const uint32_t lb = xl * 2;
+            const uint32_t ledge = (x0 >= 2 ? x0 - 2 : 0) * 2;            // dword holding cell x0-1
+            const uint32_t redge = (x0 + 128 < nx ? x0 + 128 : x0) * 2;   // dword holding cell x0+128
+            uint32_t raw[Q];
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
+                const uint32_t cur = mem.ld32(row, lb);
+                if constexpr (CX[i] == 0) {
+                    raw[i] = cur;
+                } else if constexpr (CX[i] > 0) {                        // cells (x-1, x)
+                    raw[i] = mid16(cur, wave_shr1(cur, mem.ld32(row, ledge)));
+                } else {                                                 // cells (x+1, x+2)
+                    uint32_t nxt = wave_shl1(cur, mem.ld32(row, redge));
+                    if (xl + 2 >= nx) nxt = cur >> 16;                  // outlet: cell nx-1 pulls itself
+                    raw[i] = mid16(nxt, cur);
+                }
+            });
+            uint32_t packed[Q];
+            {
+                T fl[Q];
+                static_for<0, Q>([&](auto I) {
+                    fl[I] = C::decode((uint16_t)(raw[I] & 0xFFFFu), wgt<T>(I), inv_scale);
+                });
+                collide<T>(fl, xa, a, smag, [&](auto I, T v) {
+                    packed[I] = (uint32_t)C::encode(v, wgt<T>(I), scale);
+                });
+            }
+            if (x0 == 0) {                                               // inlet cell x = 0 (:67-71)
+                T u0 = (T)a.uin[z];
+                if (a.gust) u0 = u0 * (T)a.gust[t];
+                const T usq15 = T(1.5) * (u0 * u0);
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    const uint32_t e =
+                        (uint32_t)C::encode(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
+                    packed[i] = (lane == 0) ? e : packed[i];
+                });
+                inlet_done = true;
+            }
+            {
+                T fl[Q];
+                static_for<0, Q>([&](auto I) {
+                    fl[I] = C::decode((uint16_t)(raw[I] >> 16), wgt<T>(I), inv_scale);
+                });
+                collide<T>(fl, xa + 1, a, smag, [&](auto I, T v) {
+                    packed[I] |= (uint32_t)C::encode(v, wgt<T>(I), scale) << 16;
+                });
+            }
+            if (xa < nx) {
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    mem.template st32<TL::NT>((size_t)i * NB + selfb, xa * 2, packed[i]);
+                });
+            }
+        }
+    } else if constexpr (PF) {
         // software-pipelined sweep: chunk c+1's loads are in flight during chunk c's work
         T cur[Q], nxt[Q];
         bool cur_loaded = false;
@@ -308,7 +389,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         }
     }
     // ---------------- inlet (:67-71): x = 0 fluid cell gets feq(1, (uin[z] g, 0, 0))
-    if (seg == 0 && lane == 0 && !(lrow[0] & SOLID_BIT)) {
+    if (seg == 0 && lane == 0 && !inlet_done && !(lrow[0] & SOLID_BIT)) {
         T u0 = (T)a.uin[z];
         if (a.gust) u0 = u0 * (T)a.gust[t];
         const T usq15 = T(1.5) * (u0 * u0);

@@ -72,7 +72,7 @@ class Autotuner:
         return res
 
     # ---------------------------------------------------------------- batches
-    def tune_batch(self, batch, variants=None, per_cu=(1, 2, 4), chunks=2, apply=True):
+    def tune_batch(self, batch, variants=None, per_cu=(2, 4, 6, 8, 16), chunks=2, apply=True):
         """Time batch chunks for each (variant, persistent grid) pair and keep the fastest
         on `batch`.  The members advance during tuning: tune a throwaway batch of the
         same shapes, or re-initialise the members afterwards."""

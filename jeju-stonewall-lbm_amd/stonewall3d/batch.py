@@ -107,7 +107,7 @@ class Lattice:
         return self.force[:self.t].cpu().numpy()
 
 
-def default_blocks(device=None, per_cu=2):
+def default_blocks(device=None, per_cu=8):
     props = torch.cuda.get_device_properties(device or torch.cuda.current_device())
     return int(props.multi_processor_count) * per_cu
 
@@ -116,7 +116,8 @@ class Batch:
     """A set of lattices of one dtype advanced K steps per launch sequence.
 
     K        : steps per chunk (one graph replay); every lattice's ring must be >= K.
-    blocks   : persistent workgroups per step launch (default: 2 per compute unit).
+    blocks   : persistent workgroups per step launch (default: 8 per compute unit, enough
+               for the step kernels' 6-7 resident wavefronts per SIMD).
     variant  : tile variant (sw_set_step_variant numbering); default: the library's
                current choice for this dtype.
     graph    : capture the chunk as a hipGraph (else launch it eagerly).
@@ -231,3 +232,69 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def main(argv=None):
+    """Batched run: n lattices 16H x 8H x 4H (H from {8,...,24}, porosity U[0, 0.45],
+    per-instance seeds), `steps` each, one persistent kernel per step over all of them,
+    K-step chunks replayed as a hipGraph, tile variant and grid picked by the autotuner."""
+    import argparse
+    import json
+    import sys
+    import time
+
+    from . import workloads
+    from .autotune import Autotuner
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--K", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true")
+    a = ap.parse_args(argv)
+    spec = workloads.batch_instances(a.instances, seed=a.seed, n_steps=a.steps)
+    cells = 0
+    b = Batch(a.dtype, max_instances=a.instances, K=a.K)
+    for H, por, seed, steps in spec:
+        w = workloads.porous_wall(H=H, porosity=por, seed=seed, n_steps=steps)
+        b.add(Lattice(w.flag, w.uin, dtype=a.dtype, ring=a.K, **w.sim_kwargs()).init(noise=0.02, seed=seed))
+        cells += w.cells
+    tuned = None
+    if not a.no_tune:
+        cand = [1, 3, 7] + ([8, 9] if a.dtype in ("f16", "bf16") and all(m.nx % 2 == 0 for m in b.members) else [])
+        tuned = Autotuner().tune_batch(b, variants=cand, per_cu=(4, 6, 8, 16), chunks=2)
+        print(f"[batch] tuned: variant {b.variant}, {b.blocks} workgroups", file=sys.stderr, flush=True)
+        # restart every lattice from its initial state after tuning
+        for (H, por, seed, steps), m in zip(spec, b.members):
+            m.init(noise=0.02, seed=seed)
+        b._upload()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    chunks = 0
+    while True:
+        b.run_chunk()
+        chunks += 1
+        if chunks % 16 == 0:
+            b.sync()
+            if all(m.done for m in b.members):
+                break
+            print(f"[batch] step {min(m.t for m in b.members)}/{a.steps} {time.perf_counter() - t0:.1f}s",
+                  file=sys.stderr, flush=True)
+    b.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert all(m.done for m in b.members)
+    finite = all(bool(torch.isfinite(m.force[:m.t]).all()) for m in b.members)
+    bpu = {"f32": 152, "f16": 76, "bf16": 76, "f64": 304}[a.dtype]
+    out = dict(form="batched", dtype=a.dtype, instances=a.instances, cells=cells, steps=a.steps, seconds=el,
+               aggregate_mlups=cells * a.steps / el / 1e6, achieved_gbs=cells * a.steps * bpu / el / 1e9,
+               variant=b.variant, workgroups=b.blocks, K=a.K, graph=b.use_graph, finite=finite,
+               tuning_ms_per_step={f"{k[0]}/{k[1]}": v for k, v in tuned.items()} if tuned else None)
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -269,6 +269,20 @@ class Simulation:
         return xz, xy
 
 
+    def q_frame(self, x0, ds, dims, q_scale, qmax=400.0):
+        """uint8 log-quantised max-pooled Q-criterion frame of the current state
+        (k_macro + k_qcoarse + q_frame, lbm3d.py:312-318, :367, :376-377)."""
+        if self.U is None:
+            self.U = torch.empty(4 * self.N, dtype=torch.float32, device=self.device)
+        nat.call("sw_macro", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.nx, self.ny, self.nz,
+                 self.scale, self.U.data_ptr(), None, _stream())
+        q = torch.empty(int(np.prod(dims)), dtype=torch.float32, device=self.device)
+        nat.call("sw_qcoarse", self.U.data_ptr(), self.links.data_ptr(), self.nx, self.ny, self.nz, int(x0), int(ds),
+                 int(dims[0]), int(dims[1]), int(dims[2]), q.data_ptr(), _stream())
+        q = np.clip(q.cpu().numpy().reshape(dims) * q_scale, 0, qmax)
+        return np.round(255 * np.log1p(q) / np.log1p(qmax)).astype(np.uint8)
+
+
 def bytes_per_update(dtype):
     d = nat.DTYPES[dtype] if isinstance(dtype, str) else dtype
     return BYTES_PER_UPDATE[d]
@@ -298,8 +312,8 @@ def _parser():
     ap.add_argument("--t-spinup", type=float, default=20)
     ap.add_argument("--t-avg", type=float, default=30)
     ap.add_argument("--snap-dt", type=float, default=0.25)
-    ap.add_argument("--snap3d-every", type=int, default=0,
-                    help="3D Q-criterion frames are not produced by this build; must stay 0")
+    ap.add_argument("--snap3d-every", type=int, default=2,
+                    help="store a 3D vortex (Q-criterion) frame every N snapshots (0 = off)")
     ap.add_argument("--snap3d-ds", type=int, default=0)
     ap.add_argument("--out", default="results")
     ap.add_argument("--ckpt-every", type=int, default=5000)
@@ -326,8 +340,6 @@ def main(argv=None):
     from .inflow import gust_table, make_gust, reference_setup
 
     a = _parser().parse_args(argv)
-    if a.snap3d_every:
-        raise SystemExit("--snap3d-every: 3D vortex frames are outside this build; use 0")
     H = a.H
     su = reference_setup(H=H, Re=a.Re, U0=a.U0, up=a.up, down=a.down, span=a.span, height=a.height, thick=a.thick,
                          t_spinup=a.t_spinup, t_avg=a.t_avg, snap_dt=a.snap_dt)
@@ -344,15 +356,19 @@ def main(argv=None):
     sim = Simulation(wall, su.uin, nu0=su.nu0, cs=a.Cs, x_sponge=su.x_sponge, nu_sponge=0.05, ybc=a.ybc,
                      dtype=a.dtype, gust=gust, n_steps=n_total, device=a.device)
     y_mid, z_half = ny // 2, int(round(0.5 * H)) + 1
+    ds3 = a.snap3d_ds if a.snap3d_ds > 0 else max(1, int(round(H / 13)))            # lbm3d.py:308-313
+    q_x0 = max(1, X0 - int(round(0.5 * H)))
+    q_dims = (min(int(round(9.0 * H)), nx - 1 - q_x0) // ds3, ny // ds3, min(int(round(2.5 * H)), nz - 1) // ds3)
+    QMAX, q_scale = 400.0, (H / su.UH) ** 2
     os.makedirs(a.out, exist_ok=True)
     ckpt = os.path.join(a.out, f"{a.case}.ckpt")
     sig = dict(grid=[nx, ny, nz], H=H, porosity=a.porosity, seed=a.seed, Re=a.Re, U0=a.U0, Cs=a.Cs, n_spin=n_spin,
-               n_total=n_total, snap_every=snap_every, snap3d=[a.snap3d_every, a.snap3d_ds], dtype=a.dtype,
+               n_total=n_total, snap_every=snap_every, snap3d=[a.snap3d_every, ds3], dtype=a.dtype,
                ybc=a.ybc, gust_seed=a.gust_seed)
-    snaps_xz, snaps_xy, t_start = [], [], 0
+    snaps_xz, snaps_xy, snaps_q, t_start = [], [], [], 0
     if os.path.isdir(ckpt) and not a.no_resume:
         t_start, n_acc, snaps = checkpoint.load(sim, ckpt, sig)
-        snaps_xz, snaps_xy = list(snaps["snaps_xz"]), list(snaps["snaps_xy"])
+        snaps_xz, snaps_xy, snaps_q = list(snaps["snaps_xz"]), list(snaps["snaps_xy"]), list(snaps["snaps_q"])
         print(f"[{a.case}] resumed from step {t_start} ({t_start/su.t_unit:.1f} H/U)", flush=True)
     else:
         sim.init(noise=a.noise, seed=a.seed)
@@ -361,7 +377,7 @@ def main(argv=None):
         checkpoint.save(sim, ckpt, t_next, sim.n_acc, sig,
                         dict(snaps_xz=np.array(snaps_xz, np.float16).reshape(-1, 3, nx, nz),
                              snaps_xy=np.array(snaps_xy, np.float16).reshape(-1, 3, nx, ny),
-                             snaps_q=np.zeros((0,), np.uint8)))
+                             snaps_q=np.array(snaps_q, np.uint8).reshape(-1, *q_dims)))
         print(f"[{a.case}] checkpoint saved at step {t_next}", flush=True)
 
     t0 = time.time()
@@ -375,6 +391,8 @@ def main(argv=None):
                 xz, xy = sim.slices(y_mid, z_half)
                 snaps_xz.append(xz)
                 snaps_xy.append(xy)
+                if a.snap3d_every > 0 and (k // snap_every) % a.snap3d_every == 0:
+                    snaps_q.append(sim.q_frame(q_x0, ds3, q_dims, q_scale, QMAX))
         if a.ckpt_every > 0 and (t + 1) % a.ckpt_every == 0 and t + 1 < n_total:
             save_ckpt(t + 1)
         if t % 1000 == 0 or t == n_total - 1:
@@ -412,8 +430,10 @@ def main(argv=None):
         snaps_xz=np.array(snaps_xz, np.float16).reshape(-1, 3, nx, nz),
         snaps_xy=np.array(snaps_xy, np.float16).reshape(-1, 3, nx, ny),
         wall_xz=wall[:, y_mid, :], wall_xy=wall[:, :, z_half], wall_front=wall.any(0), wall_top=wall.any(2),
-        wall_ds2=wall[::2, ::2, ::2], snaps_q=np.zeros((0, 0, 0, 0), np.uint8),
+        wall_ds2=wall[::2, ::2, ::2], snaps_q=np.array(snaps_q, dtype=np.uint8).reshape(-1, *q_dims),
         meta=json.dumps(dict(case=a.case, porosity_target=a.porosity, porosity=float(phi), H=H, T=T, X0=X0,
+                             q3d=dict(x0=q_x0, ds=ds3, dims=list(q_dims), qmax=QMAX,
+                                      every=a.snap3d_every * snap_every),
                              nx=nx, ny=ny, nz=nz, UH=su.UH, U0=a.U0, Re=a.Re, nu=su.nu0, Cs=a.Cs, n_spin=n_spin,
                              n_avg=n_total - n_spin, steps_per_unit=su.t_unit, snap_every=snap_every, y_mid=y_mid,
                              z_half=z_half, seed=a.seed, device=a.device, dtype=a.dtype, ybc=a.ybc,

@@ -7,6 +7,8 @@
                                 inlet 0.05 (z/H)^(1/7) capped at 0.06, nu = 2e-4, Cs = 0.16
   porous_wall(H, porosity, ...) 16H x 8H x 4H, wall at x = 4H, thickness 0.6H, rounded-box
                                 stones opened to a void fraction, gusting inlet, nu = 2e-4
+                                (gust_time_scale s evaluates the gust at s*t: a lattice at
+                                half the resolution uses s = 2 to keep physical time)
                                 (H = 40, 20 %: the 640 x 320 x 160 headline lattice)
   batch_instances / serving_requests  the uneven-lattice mixes of the batched and serving forms
 Boundaries of the explicit cases: no-slip floor, free-slip top and sides, zero-gradient
@@ -67,15 +69,17 @@ def box_c0(n_steps=2000, ybc="freeslip", sponge=True, nx=160, ny=80, nz=40, H=10
 
 
 def porous_wall(H=40, porosity=0.2, seed=0, gust_seed=None, n_steps=20000, nu0=2e-4, sponge=True, ybc="freeslip",
-                dims=(16, 8, 4)):
+                dims=(16, 8, 4), gust_time_scale=1.0):
     nx, ny, nz = dims[0] * H, dims[1] * H, dims[2] * H
     X0, T = 4 * H, max(2, int(round(0.6 * H)))
     wall, info = geometry.rounded_box_wall(nx, ny, nz, X0, T, H, porosity, seed)
     uin = baseline_profile(nz, H)
     g = make_gust(seed if gust_seed is None else gust_seed)
     info.update(X0=X0, T=T, gust=dict(a=g.a.tolist(), f=g.f.tolist(), phi=g.phi.tolist(), amp=g.amp))
+    gt = gust_table(g, n_steps) if gust_time_scale == 1.0 else \
+        g.factor(np.arange(n_steps) * float(gust_time_scale)).astype(np.float32)
     return Workload(f"porous wall H={H} void={porosity} {nx}x{ny}x{nz}", flag_from_mask(wall), uin, H, 0.05, nu0,
-                    0.16, nx - 3 * H if sponge else None, 0.05, ybc, gust_table(g, n_steps), n_steps, 0, info)
+                    0.16, nx - 3 * H if sponge else None, 0.05, ybc, gt, n_steps, 0, info)
 
 
 def batch_instances(n=64, seed=0, heights=(8, 12, 16, 20, 24), n_steps=5000):

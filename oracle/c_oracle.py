@@ -38,6 +38,7 @@ def lib():
         L.oracle_f32_to_bf16.restype = ctypes.c_uint16
         L.oracle_bf16_to_f32.argtypes = [ctypes.c_uint16]
         L.oracle_bf16_to_f32.restype = f
+        L.oracle_qcoarse.argtypes = [p, p, i, i, i, i, i, i, i, i, p]
         L.oracle_set_threads.argtypes = [i]
         L.oracle_num_threads.restype = i
         L.oracle_tables.argtypes = [p] * 7
@@ -127,3 +128,14 @@ def decode(f, storage, scale=1.0):
     else:
         d = (f.astype(np.uint32) << 16).view(np.float32)
     return w + d * np.float32(1.0 / scale)
+
+
+def qcoarse(U, flag, x0, ds, dims):
+    """k_qcoarse (lbm3d.py:224-238) on U (4, nx, ny, nz) float32."""
+    U = np.ascontiguousarray(U, np.float32)
+    flag = np.ascontiguousarray(flag, np.uint8)
+    nx, ny, nz = flag.shape
+    out = np.zeros(dims, np.float32)
+    lib().oracle_qcoarse(_ptr(U), _ptr(flag), nx, ny, nz, int(x0), int(ds), int(dims[0]), int(dims[1]), int(dims[2]),
+                         _ptr(out))
+    return out

@@ -478,3 +478,49 @@ void oracle_set_threads(int n) {
     (void)n;
 #endif
 }
+
+/* ================================================================ Q-criterion frames
+ * q_at + k_qcoarse (lbm3d.py:203-238): max-pooled positive Q = -1/2 du_i/dx_j du_j/dx_i
+ * (central differences; 0 next to solids) over ds^3 blocks of the crop x >= x0, y >= 0,
+ * z >= 0.  U is (4, nx, ny, nz) float32 as k_macro writes it; out is (qx, qy, qz). */
+static float q_at(const float* U, const uint8_t* flag, int x, int y, int z, int nx, int ny, int nz) {
+    const size_t N = (size_t)nx * ny * nz;
+    int yp = (y + 1) % ny, ym = (y - 1 + ny) % ny;
+    if (flag[FIDX(x, y, z)] || flag[FIDX(x + 1, y, z)] || flag[FIDX(x - 1, y, z)] || flag[FIDX(x, yp, z)] ||
+        flag[FIDX(x, ym, z)] || flag[FIDX(x, y, z + 1)] || flag[FIDX(x, y, z - 1)])
+        return 0.0f;
+    const float* u1 = U + N;
+    const float* u2 = U + 2 * N;
+    const float* u3 = U + 3 * N;
+    float a11 = 0.5f * (u1[FIDX(x + 1, y, z)] - u1[FIDX(x - 1, y, z)]);
+    float a12 = 0.5f * (u1[FIDX(x, yp, z)] - u1[FIDX(x, ym, z)]);
+    float a13 = 0.5f * (u1[FIDX(x, y, z + 1)] - u1[FIDX(x, y, z - 1)]);
+    float a21 = 0.5f * (u2[FIDX(x + 1, y, z)] - u2[FIDX(x - 1, y, z)]);
+    float a22 = 0.5f * (u2[FIDX(x, yp, z)] - u2[FIDX(x, ym, z)]);
+    float a23 = 0.5f * (u2[FIDX(x, y, z + 1)] - u2[FIDX(x, y, z - 1)]);
+    float a31 = 0.5f * (u3[FIDX(x + 1, y, z)] - u3[FIDX(x - 1, y, z)]);
+    float a32 = 0.5f * (u3[FIDX(x, yp, z)] - u3[FIDX(x, ym, z)]);
+    float a33 = 0.5f * (u3[FIDX(x, y, z + 1)] - u3[FIDX(x, y, z - 1)]);
+    return -0.5f * (a11 * a11 + a22 * a22 + a33 * a33 + 2.0f * (a12 * a21 + a13 * a31 + a23 * a32));
+}
+
+int oracle_qcoarse(const float* U, const uint8_t* flag, int nx, int ny, int nz, int x0, int ds, int qx, int qy,
+                   int qz, float* out) {
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < qx; i++)
+        for (int j = 0; j < qy; j++)
+            for (int k = 0; k < qz; k++) {
+                float best = 0.0f;
+                for (int a = 0; a < ds; a++)
+                    for (int b = 0; b < ds; b++)
+                        for (int c = 0; c < ds; c++) {
+                            int x = x0 + i * ds + a, y = j * ds + b, z = k * ds + c;
+                            if (x >= 1 && x < nx - 1 && z >= 1 && z < nz - 1) {
+                                float q = q_at(U, flag, x, y, z, nx, ny, nz);
+                                best = q > best ? q : best;
+                            }
+                        }
+                out[((size_t)i * qy + j) * qz + k] = best;
+            }
+    return 0;
+}

@@ -23,9 +23,10 @@ pytestmark = pytest.mark.gpu
 SCALE = {"f32": 1.0, "f16": 1024.0, "bf16": 1.0, "f64": 1.0}
 
 
-def small_case(nx=48, ny=13, nz=18, H=8, seed=3):
-    wall, _ = geometry.rounded_box_wall(nx, ny, nz, 12, 5, H, 0.3, seed, size=(0.4, 0.8))
-    wall[30, 4, 3] = True                                # a lone stone
+def small_case(nx=48, ny=13, nz=18, H=8, seed=3, X0=12, lone=True):
+    wall, _ = geometry.rounded_box_wall(nx, ny, nz, X0, 5, H, 0.3, seed, size=(0.4, 0.8))
+    if lone:
+        wall[30, 4, 3] = True                            # a lone stone
     flag = flag_from_mask(wall)
     uin = baseline_profile(nz, H)
     return flag, uin
@@ -231,18 +232,58 @@ def test_cli_short_run(gpu, tmp_path):
     assert not os.path.isdir(os.path.join(out, "q.ckpt"))
 
 
-@pytest.mark.parametrize("variant", range(8))
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_every_tile_variant_bit_identical(gpu, variant, storage):
-    """Every kernel variant the autotuner may pick is bit-identical to the oracle."""
+@pytest.mark.parametrize("variant", range(10))
+@pytest.mark.parametrize("storage,nx", [("f32", 200), ("f16", 200), ("bf16", 200), ("f16", 256), ("bf16", 77)])
+def test_every_tile_variant_bit_identical(gpu, variant, storage, nx):
+    """Every kernel variant the autotuner may pick is bit-identical to the oracle (row
+    lengths with a partial last chunk, whole 128-cell chunks, and an odd length)."""
     from stonewall3d import _native as nat
     old = nat.lib().sw_get_step_variant(nat.DTYPES[storage])
     nat.call("sw_set_step_variant", nat.DTYPES[storage], variant)
     try:
-        flag, uin = small_case(nx=200, ny=9, nz=14)
+        flag, uin = small_case(nx=nx, ny=9, nz=14)
         sim, f, F = run_both(flag, uin, 12, storage, 1, gust=gust_table(make_gust(7), 12))
         g = sim.populations()
         assert not np.where(fluid_mask(flag), g != f, False).any()
         assert np.allclose(sim.force_rows(), F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
     finally:
         nat.call("sw_set_step_variant", nat.DTYPES[storage], old)
+
+
+@pytest.mark.parametrize("variant", [8, 9])
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_pair_path_inlet_chunk(gpu, variant, storage):
+    """16-bit pair tiles when the first 128-cell chunk is free of solids: the inlet cell is
+    written inside the packed 32-bit store."""
+    from stonewall3d import _native as nat
+    old = nat.lib().sw_get_step_variant(nat.DTYPES[storage])
+    nat.call("sw_set_step_variant", nat.DTYPES[storage], variant)
+    try:
+        flag, uin = small_case(nx=384, ny=7, nz=12, X0=200, lone=False)
+        sim, f, F = run_both(flag, uin, 10, storage, 1, gust=gust_table(make_gust(4), 10))
+        assert not np.where(fluid_mask(flag), sim.populations() != f, False).any()
+        assert np.allclose(sim.force_rows(), F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
+    finally:
+        nat.call("sw_set_step_variant", nat.DTYPES[storage], old)
+
+
+def test_q_frame_bit_identical(gpu):
+    """Max-pooled Q-criterion frames (k_qcoarse, lbm3d.py:203-238) equal the oracle's on the
+    same macroscopic field."""
+    w = workloads.reference(H=10, porosity=0.2, seed=7, n_steps=40)
+    sim = Simulation(w.flag, w.uin, **w.sim_kwargs()).init(noise=0.02, seed=7)
+    sim.step(40)
+    U = sim.macro()
+    nx, ny, nz = w.shape
+    ds, x0 = 1, max(1, w.info["X0"] - 5)
+    dims = (min(90, nx - 1 - x0) // ds, ny // ds, min(25, nz - 1) // ds)
+    sim_q = torch.empty(int(np.prod(dims)), dtype=torch.float32, device="cuda")
+    from stonewall3d import _native as nat
+    nat.call("sw_qcoarse", sim.U.data_ptr(), sim.links.data_ptr(), nx, ny, nz, x0, ds, *dims, sim_q.data_ptr(),
+             torch.cuda.current_stream().cuda_stream)
+    q_gpu = sim_q.cpu().numpy().reshape(dims)
+    q_ref = co.qcoarse(U, w.flag, x0, ds, dims)
+    assert np.array_equal(q_gpu, q_ref)
+    assert q_ref.max() > 0
+    frame = sim.q_frame(x0, 2, (dims[0] // 2, dims[1] // 2, dims[2] // 2), (10 / w.UH) ** 2)
+    assert frame.dtype == np.uint8 and frame.max() > 0

@@ -82,3 +82,12 @@ def test_build_wall_matches_reference_masks(case):
     assert np.array_equal(m[x0:x1], ref)
     assert phi == case["phi"]
     assert int(m.sum()) == case["solid_cells"]
+
+
+def test_vortex_frame_dims_match_published():
+    """The Q-frame crop of lbm3d.py:308-310 at H=40 gives the published vortex dims."""
+    s = reference_setup(H=40)
+    ds3 = max(1, int(round(40 / 13)))
+    q_x0 = max(1, s.X0 - int(round(0.5 * 40)))
+    dims = [min(int(round(9.0 * 40)), s.nx - 1 - q_x0) // ds3, s.ny // ds3, min(int(round(2.5 * 40)), s.nz - 1) // ds3]
+    assert dims == REF["cases"][0]["vortex_dims"] == [120, 53, 33]
