@@ -233,7 +233,7 @@ using namespace sw;
 
 // Step variants (the autotuner's choices, per dtype): the tile list of sw_step.hpp.
 // Lattices whose byte offsets exceed 32 bits always use variant 0 with 64-bit addressing.
-static int g_step_variant[4] = {7, 1, 1, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
+static int g_step_variant[4] = {7, 8, 8, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
 
 template <class S, class TL, bool BUF>
 static void launch_step_v(const StepArgs& a, hipStream_t s) {
@@ -250,7 +250,7 @@ template <class S>
 static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
     if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV0, false>(a, s); return; }
     int v = g_step_variant[dtype];
-    if ((v == 8 || v == 9) && (sizeof(S) != 2 || (a.nx & 1))) v = 1;     // pairs: 16-bit storage, even nx
+    if ((v == 8 || v == 9) && sizeof(S) != 2) v = 1;    // pairs: 16-bit storage only (odd nx falls back in-kernel)
     switch (v) {
 #define SW_CASE(id, TL) case id: launch_step_v<S, TL, true>(a, s); break;
         SW_TILE_LIST(SW_CASE)

@@ -260,8 +260,39 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         collide_store(x, fl);
     };
 
+    // one-cell-per-lane sweep of chunks [cbeg, nchunks)
+    auto plain_sweep = [&]() {
+#pragma clang loop unroll(disable)
+        for (uint32_t c0 = cbeg; c0 < nchunks; c0 += CPL) {
+            bool special = false;
+#pragma unroll
+            for (int k = 0; k < CPL; k++)
+                if (c0 + k < nchunks) special |= special_chunk(c0 + k);
+            if (!special) {
+                // ---------------- fast path: no solid cell and no solid neighbour in the chunk(s)
+                T fl[CPL][Q];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    const uint32_t x = (c0 + k) * 64 + lane;
+                    if (x < nx && x != 0) load_fast(x, fl[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    const uint32_t x = (c0 + k) * 64 + lane;
+                    if (x < nx && x != 0) collide_store(x, fl[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < CPL; k++) general_cell((c0 + k) * 64 + lane);
+            }
+        }
+    };
+
     bool inlet_done = false;
     if constexpr (TL::PAIR && ES == 2) {
+      if (nx & 1u) {
+        plain_sweep();                    // pairs need an even row length
+      } else {
         // 128-cell chunks; lane L owns cells x0 + 2L and x0 + 2L + 1 (nx is even)
         const uint32_t npc = (nx + 127) / 128;
         const uint32_t plen = (npc + TL::NSEG - 1) / TL::NSEG;
@@ -334,6 +365,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                 });
             }
         }
+      }
     } else if constexpr (PF) {
         // software-pipelined sweep: chunk c+1's loads are in flight during chunk c's work
         T cur[Q], nxt[Q];
@@ -363,30 +395,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             cur_loaded = next_fast;
         }
     } else {
-#pragma clang loop unroll(disable)
-        for (uint32_t c0 = cbeg; c0 < nchunks; c0 += CPL) {
-            bool special = false;
-#pragma unroll
-            for (int k = 0; k < CPL; k++)
-                if (c0 + k < nchunks) special |= special_chunk(c0 + k);
-            if (!special) {
-                // ---------------- fast path: no solid cell and no solid neighbour in the chunk(s)
-                T fl[CPL][Q];
-#pragma unroll
-                for (int k = 0; k < CPL; k++) {
-                    const uint32_t x = (c0 + k) * 64 + lane;
-                    if (x < nx && x != 0) load_fast(x, fl[k]);
-                }
-#pragma unroll
-                for (int k = 0; k < CPL; k++) {
-                    const uint32_t x = (c0 + k) * 64 + lane;
-                    if (x < nx && x != 0) collide_store(x, fl[k]);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < CPL; k++) general_cell((c0 + k) * 64 + lane);
-            }
-        }
+        plain_sweep();
     }
     // ---------------- inlet (:67-71): x = 0 fluid cell gets feq(1, (uin[z] g, 0, 0))
     if (seg == 0 && lane == 0 && !inlet_done && !(lrow[0] & SOLID_BIT)) {
