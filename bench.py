@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--porosity", type=float, default=0.2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy one FP32 population array (19 x 640 x 320 x 160 x 4 B) 5 times: the known-size "
+                         "copy against which rocprofv3's FETCH_SIZE / WRITE_SIZE counters are scaled")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the stone-wall step runs on a single MI355X here "
@@ -76,6 +79,15 @@ def main():
 
     import numpy as np
     import torch
+    if a.calibrate_copy:
+        n = 19 * 640 * 320 * 160
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}))
+        return
     from stonewall3d import workloads
     from stonewall3d.lbm3d import Simulation, bytes_per_update
 

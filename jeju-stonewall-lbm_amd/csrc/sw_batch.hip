@@ -54,18 +54,20 @@ __device__ __forceinline__ StepArgs instance_args(const sw_instance& I, int k) {
 }
 
 template <class S, class TL, bool BUF>
-__global__ __launch_bounds__(64 * TL::WPB, TL::OCC) void k_batch_step(const void* table, int k) {
+__global__ __launch_bounds__(64 * TL::WPB, TL::OCC) void k_batch_step(const void* __restrict__ table, int k) {
     const TableHeader* h = static_cast<const TableHeader*>(table);
     const sw_instance* e = table_entries(table);
-    const int n = h->n, total = h->total_items;
+    const int n = __builtin_amdgcn_readfirstlane(h->n), total = __builtin_amdgcn_readfirstlane(h->total_items);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // a workgroup's items ascend, so after one search its instance only moves forward
+    int i = total > 0 ? __builtin_amdgcn_readfirstlane(find_instance(e, n, blockIdx.x)) : 0;
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const int i = find_instance(e, n, item);
+        while (i + 1 < n && __builtin_amdgcn_readfirstlane(e[i + 1].item0) <= item) i++;
         const sw_instance& I = e[i];
-        const int t = I.t + k;
-        if (t >= I.t_end) continue;                          // finished: uniform over the workgroup
-        const StepArgs a = instance_args(I, k);
-        const int local = item - I.item0;
+        const int t = __builtin_amdgcn_readfirstlane(I.t) + k;
+        if (t >= __builtin_amdgcn_readfirstlane(I.t_end)) continue;   // finished: uniform
+        const StepArgs a = wave_uniform(instance_args(I, k));
+        const int local = item - __builtin_amdgcn_readfirstlane(I.item0);
         const int z = local / a.nyb, yb = local - z * a.nyb;
         const int y = yb * ROWS + (wave % ROWS);
         double fx = 0.0, fz = 0.0;

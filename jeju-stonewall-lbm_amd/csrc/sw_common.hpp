@@ -264,6 +264,22 @@ __device__ __forceinline__ uint32_t mid16(uint32_t hi, uint32_t lo) {
     return __builtin_amdgcn_alignbit(hi, lo, 16);
 }
 
+// A value every lane of the wavefront holds identically, moved to scalar registers word by
+// word.  Values loaded from device memory (e.g. an instance table) are not known to be
+// uniform to the compiler; without this, buffer descriptors built from them sit in vector
+// registers and every buffer access becomes a readfirstlane loop.
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (unsigned j = 0; j < sizeof(T) / 4; j++) w[j] = __builtin_amdgcn_readfirstlane(w[j]);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
 // ------------------------------------------------------------- wave reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

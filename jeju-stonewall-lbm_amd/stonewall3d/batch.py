@@ -264,7 +264,7 @@ def main(argv=None):
     tuned = None
     if not a.no_tune:
         cand = [1, 3, 7] + ([8, 9] if a.dtype in ("f16", "bf16") and all(m.nx % 2 == 0 for m in b.members) else [])
-        tuned = Autotuner().tune_batch(b, variants=cand, per_cu=(4, 6, 8, 16), chunks=2)
+        tuned = Autotuner().tune_batch(b, variants=cand, per_cu=(4, 8, 16, 32), chunks=2)
         print(f"[batch] tuned: variant {b.variant}, {b.blocks} workgroups", file=sys.stderr, flush=True)
         # restart every lattice from its initial state after tuning
         for (H, por, seed, steps), m in zip(spec, b.members):

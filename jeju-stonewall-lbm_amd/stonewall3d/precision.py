@@ -21,14 +21,16 @@ from . import stats, workloads
 from .lbm3d import Simulation
 
 
-def run_case(H, porosity, seed, dtype, spin, avg, acc_every=5, log=None):
+def run_case(H, porosity, seed, dtype, spin, avg, acc_every=5, log=None, noise_seed=None, cs=None):
     UH = 0.05
     t_unit = H / UH
     n_spin, n_avg = int(spin * t_unit), int(avg * t_unit)
     n = n_spin + n_avg
     w = workloads.porous_wall(H=H, porosity=porosity, seed=seed, n_steps=n, gust_time_scale=40.0 / H)
+    if cs is not None:
+        w.cs = float(cs)
     sim = Simulation(w.flag, w.uin, dtype=dtype, **w.sim_kwargs())
-    sim.init(noise=0.02, seed=seed)
+    sim.init(noise=0.02, seed=seed if noise_seed is None else noise_seed)
     t0 = time.perf_counter()
     sim.step(n_spin)
     k = 0
@@ -46,7 +48,7 @@ def run_case(H, porosity, seed, dtype, spin, avg, acc_every=5, log=None):
     st = stats.force_stats(Cd, Cm, Cl, n_spin)
     mean = sim.mean_field()
     area = stats.recirculation_area(mean[1], w.flag == 1, H, w.info["X0"], UH, w.shape[1])
-    out = dict(dtype=dtype, H=H, porosity=porosity, seed=seed, cells=w.cells, steps=n, n_spin=n_spin,
+    out = dict(dtype=dtype, H=H, porosity=porosity, seed=seed, noise_seed=noise_seed, cs=w.cs, cells=w.cells, steps=n, n_spin=n_spin,
                recirculation_area=area, finite=bool(np.isfinite(Cd).all()), seconds=wall_s, **st)
     del sim
     torch.cuda.empty_cache()
@@ -68,17 +70,27 @@ def main(argv=None):
     ap.add_argument("--avg", type=float, default=20.0, help="averaging window [H/U]")
     ap.add_argument("--dtypes", default="f64,f32,f16,bf16")
     ap.add_argument("--half-res", action="store_true", help="also run FP64 at H/2")
+    ap.add_argument("--cs", type=float, default=None, help="Smagorinsky constant (default 0.16)")
+    ap.add_argument("--noise-seeds", default="", help="extra runs that differ only in the init-noise seed, "
+                                                     "as dtype:seed,... (run-to-run spread of the statistics)")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     runs = {}
     for dt in a.dtypes.split(","):
-        runs[dt] = run_case(a.H, a.porosity, a.seed, dt, a.spin, a.avg, log=log)
+        runs[dt] = run_case(a.H, a.porosity, a.seed, dt, a.spin, a.avg, log=log, cs=a.cs)
         log(json.dumps(runs[dt]))
     out = dict(runs=runs)
     if "f64" in runs:
         out["vs_f64"] = {dt: compare(r, runs["f64"]) for dt, r in runs.items() if dt != "f64"}
+    if a.noise_seeds:
+        out["noise_runs"] = []
+        for spec in a.noise_seeds.split(","):
+            dt, ns = spec.split(":")
+            r = run_case(a.H, a.porosity, a.seed, dt, a.spin, a.avg, log=log, noise_seed=int(ns), cs=a.cs)
+            log(json.dumps(r))
+            out["noise_runs"].append(r)
     if a.half_res:
-        half = run_case(a.H // 2, a.porosity, a.seed, "f64", a.spin, a.avg, log=log)
+        half = run_case(a.H // 2, a.porosity, a.seed, "f64", a.spin, a.avg, log=log, cs=a.cs)
         out["f64_half_res"] = half
         out["vs_f64_half_res"] = {dt: compare(r, half) for dt, r in runs.items()}
     print(json.dumps(out), flush=True)

@@ -82,7 +82,8 @@ class Server:
         self.keep_force = keep_force
         self.noise = float(noise)
         self.device = torch.device(device)
-        self.variants = variants or {}
+        self.variants = dict(variants or {})
+        self.blocks = {}
         self.batches = {}
         self.streams = {}
         self.used = 0.0
@@ -94,8 +95,8 @@ class Server:
         if dtype not in self.batches:
             self.streams[dtype] = torch.cuda.Stream(device=self.device)
             with torch.cuda.stream(self.streams[dtype]):
-                self.batches[dtype] = Batch(dtype, self.max_instances, self.K, graph=self.graph, device=self.device,
-                                            variant=self.variants.get(dtype))
+                self.batches[dtype] = Batch(dtype, self.max_instances, self.K, blocks=self.blocks.get(dtype),
+                                            graph=self.graph, device=self.device, variant=self.variants.get(dtype))
         return self.batches[dtype]
 
     def make_lattice(self, req):
@@ -115,40 +116,71 @@ class Server:
                       time.perf_counter() - t0, float(cd.mean()), float(cd.max()), cs,
                       F if self.keep_force else None)
 
+    def tune(self, dtypes=("f32", "f16"), heights=(16, 24, 32), tuner=None):
+        """Pick each dtype's tile variant and persistent grid with the autotuner, on a
+        throwaway batch of representative lattices (tuning advances its lattices, so real
+        requests never take part in it).  Returns {dtype: (variant, workgroups)}."""
+        from .autotune import Autotuner
+        tuner = tuner or Autotuner()
+        out = {}
+        for d in dtypes:
+            b = Batch(d, max_instances=len(heights), K=self.K, graph=self.graph, device=self.device)
+            for h in heights:
+                w = workloads.porous_wall(H=h, porosity=0.2, seed=h, n_steps=4 * self.K)
+                b.add(Lattice(w.flag, w.uin, dtype=d, ring=self.K, device=self.device, **w.sim_kwargs())
+                      .init(noise=self.noise, seed=h))
+            cand = [1, 3, 7] + ([8, 9] if d in ("f16", "bf16") else [])
+            tuner.tune_batch(b, variants=cand, per_cu=(4, 8, 16, 32), chunks=2)
+            self.variants[d], self.blocks[d] = b.variant, b.blocks
+            out[d] = (b.variant, b.blocks)
+            b.close()
+            del b
+        torch.cuda.synchronize(self.device)
+        return out
+
+    def _admit(self, queue, t0):
+        """Admit queued requests in FIFO order while the budget and the batch slots allow
+        (a request that does not fit blocks those behind it).  Geometry is built on the
+        host and the lattice prepared on the side stream, so this overlaps chunks already
+        in flight; joining a batch waits for that batch's chunk to end."""
+        while queue:
+            req = queue[0]
+            need = lattice_bytes(req.H, req.dtype, req.steps, self.K)
+            b = self._batch(req.dtype)
+            if self.used + need > self.budget or len(b.members) >= self.max_instances:
+                if self.used == 0 and not b.members:
+                    raise RuntimeError(f"request {req.rid} needs {need/1e9:.1f} GB > budget")
+                return
+            queue.pop(0)
+            with torch.cuda.stream(self.prep_stream):
+                lat = self.make_lattice(req)
+            self.prep_stream.synchronize()
+            lat.admitted = time.perf_counter()
+            lat.reserved = need
+            self.used += need
+            with torch.cuda.stream(self.streams[req.dtype]):
+                b.add(lat)
+
     def serve(self, requests, realtime=True, log=None):
-        """Run a finite trace; returns (results, report)."""
+        """Run a finite trace; returns (results, report).
+
+        Each iteration launches one K-step chunk per non-empty dtype batch (asynchronous,
+        one stream per dtype, so the batches overlap on the device), admits the requests
+        that have arrived while those chunks run, then retires the lattices that
+        finished."""
         pending = sorted(requests, key=lambda r: r.arrival)
         queue, results = [], []
         t0 = time.perf_counter()
         busy = 0.0
         cell_updates = 0
         while pending or queue or any(b.members for b in self.batches.values()):
-            now = time.perf_counter() - t0
-            if not realtime:
-                now = float("inf")
+            now = time.perf_counter() - t0 if realtime else float("inf")
             while pending and pending[0].arrival <= now:
                 queue.append(pending.pop(0))
-            # admission (FIFO; a request that does not fit blocks those behind it)
-            while queue:
-                req = queue[0]
-                need = lattice_bytes(req.H, req.dtype, req.steps, self.K)
-                b = self._batch(req.dtype)
-                if self.used + need > self.budget or len(b.members) >= self.max_instances:
-                    if self.used == 0 and not b.members:
-                        raise RuntimeError(f"request {req.rid} needs {need/1e9:.1f} GB > budget")
-                    break
-                queue.pop(0)
-                with torch.cuda.stream(self.prep_stream):
-                    lat = self.make_lattice(req)
-                self.prep_stream.synchronize()
-                lat.admitted = time.perf_counter()
-                lat.reserved = need
-                self.used += need
-                with torch.cuda.stream(self.streams[req.dtype]):
-                    b.add(lat)
             active = [b for b in self.batches.values() if b.members]
             if not active:
-                if pending:
+                self._admit(queue, t0)
+                if not any(b.members for b in self.batches.values()) and pending:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
                 continue
             ts = time.perf_counter()
@@ -157,6 +189,7 @@ class Server:
                     cell_updates += m.N * max(0, min(self.K, m.n_steps - m.t))
                 with torch.cuda.stream(self.streams[nat_name(b.dtype)]):
                     b.run_chunk()
+            self._admit(queue, t0)
             for b in active:
                 with torch.cuda.stream(self.streams[nat_name(b.dtype)]):
                     finished = b.retire_finished()
@@ -200,12 +233,21 @@ def main(argv=None):
     ap.add_argument("--budget-gb", type=float, default=250.0)
     ap.add_argument("--K", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the library's default tiles and 8 workgroups/CU")
     a = ap.parse_args(argv)
     reqs = trace(a.requests, a.rate, a.seed, a.steps_scale, (a.hmin, a.hmax))
     srv = Server(budget_bytes=a.budget_gb * 1e9, K=a.K)
+    tuned = None if a.no_tune else srv.tune(sorted({r.dtype for r in reqs}))
     results, rep = srv.serve(reqs)
-    offered = sum(512 * r.H ** 3 * r.steps for r in reqs) / max(reqs[-1].arrival, 1e-9) / 1e6
-    rep.update(rate=a.rate, steps_scale=a.steps_scale, offered_mlups=offered)
+    work = sum(512 * r.H ** 3 * r.steps for r in reqs)
+    rep.update(rate=a.rate, steps_scale=a.steps_scale, K=a.K, budget_gb=a.budget_gb,
+               offered_mlups=work / max(reqs[-1].arrival, 1e-9) / 1e6,
+               arrival_span_s=reqs[-1].arrival, tuned=tuned,
+               by_dtype={d: dict(requests=sum(r.dtype == d for r in results),
+                                 p50_latency_s=float(np.percentile([r.latency for r in results if r.dtype == d], 50)),
+                                 p99_latency_s=float(np.percentile([r.latency for r in results if r.dtype == d], 99)))
+                         for d in sorted({r.dtype for r in results})},
+               all_finite=bool(all(np.isfinite(r.cd_mean) for r in results)))
     print(json.dumps(rep), flush=True)
     srv.close()
 
