@@ -34,11 +34,9 @@ def traffic_from_profile(dtype):
     return None
 
 
-def cpu_baseline(w, budget_s=15.0, max_steps=20):
-    """FP32 oracle on the host: as many steps of the same lattice as fit in ~budget_s."""
+def _oracle_rate(co, w, threads, budget_s, max_steps):
+    """Steps of the FP32 oracle on the host until budget_s or max_steps: (MLUPS, steps, s)."""
     import numpy as np
-    from oracle import c_oracle as co
-    threads = min(16, os.cpu_count() or 1)
     co.set_threads(threads)
     f = co.init(w.flag, w.uin, "f32")
     g = np.empty_like(f)
@@ -53,9 +51,23 @@ def cpu_baseline(w, budget_s=15.0, max_steps=20):
             break
     el = time.perf_counter() - t0
     nx, ny, nz = w.flag.shape
-    return dict(value=steps * nx * ny * nz / el / 1e6, unit="MLUPS", cores=threads, kind="port",
+    return steps * nx * ny * nz / el / 1e6, steps, el
+
+
+def cpu_baseline(w, budget_s=15.0, max_steps=40):
+    """FP32 oracle (a C restatement of the reference kernel) on the host, on the same
+    lattice: as many steps as fit in ~budget_s with one OpenMP thread per core (at most 16),
+    plus a one-thread sample of ~budget_s / 2."""
+    from oracle import c_oracle as co
+    threads = min(16, os.cpu_count() or 1)
+    v, steps, el = _oracle_rate(co, w, threads, budget_s, max_steps)
+    v1, steps1, el1 = _oracle_rate(co, w, 1, budget_s / 2, 2)
+    nx, ny, nz = w.flag.shape
+    return dict(value=v, unit="MLUPS", cores=threads, kind="port",
                 sample=f"{steps} steps of the {nx}x{ny}x{nz} lattice, FP32 C oracle, {threads} OpenMP threads, "
-                       f"{el:.1f} s")
+                       f"{el:.1f} s",
+                single_thread=dict(value=v1, unit="MLUPS", cores=1,
+                                   sample=f"{steps1} step(s) of the same lattice, one thread, {el1:.1f} s"))
 
 
 def main():

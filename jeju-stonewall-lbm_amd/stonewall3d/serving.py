@@ -14,6 +14,7 @@ reduction are per lattice and in a fixed order (bit-identical alone and in the m
 """
 import argparse
 import json
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -238,7 +239,9 @@ def main(argv=None):
     reqs = trace(a.requests, a.rate, a.seed, a.steps_scale, (a.hmin, a.hmax))
     srv = Server(budget_bytes=a.budget_gb * 1e9, K=a.K)
     tuned = None if a.no_tune else srv.tune(sorted({r.dtype for r in reqs}))
-    results, rep = srv.serve(reqs)
+    results, rep = srv.serve(reqs, log=lambda r: print(
+        f"[serve] request {r.rid} done: {r.dtype} H={r.H} {r.steps} steps, latency {r.latency:.2f} s, "
+        f"Cd {r.cd_mean:.4f}", file=sys.stderr, flush=True))
     work = sum(512 * r.H ** 3 * r.steps for r in reqs)
     rep.update(rate=a.rate, steps_scale=a.steps_scale, K=a.K, budget_gb=a.budget_gb,
                offered_mlups=work / max(reqs[-1].arrival, 1e-9) / 1e6,
