@@ -1,7 +1,7 @@
 """Inlet profiles, the gust factor and the derived run constants.
 
 reference_setup() restates the grid / inlet / viscosity / step-count set-up of
-stonewall3d/lbm3d.py:270-282; baseline_profile() and gust_table() are the inflow of the
+stonewall3d/lbm3d.py:270-282; power_law_profile() and gust_table() are the inflow of the
 explicit workloads (power law 0.05 (z/H)^(1/7) capped at 0.06, and a gust factor
 1 + 0.25 sum_k a_k sin(2 pi f_k t + phi_k)), which the reference does not have.
 """
@@ -47,7 +47,7 @@ def reference_setup(H=40, Re=15000.0, U0=0.08, up=4.0, down=16.0, span=4.0, heig
                     int(nx - 3 * H))
 
 
-def baseline_profile(nz, H, u_ref=0.05, cap=0.06):
+def power_law_profile(nz, H, u_ref=0.05, cap=0.06):
     """u(zeta) = u_ref (zeta/H)^(1/7), capped at `cap`; zeta = k - 1/2; the ground cell gets 0."""
     zeta = np.arange(nz) - 0.5
     u = u_ref * np.clip(zeta / H, 0, None) ** (1 / 7)

@@ -3,7 +3,7 @@
   reference(H, porosity, seed)  lbm3d.py's default case (grid 20H x 4H x (8H+1), wall at
                                 x = 4H, thickness 0.3H, periodic y, U0 = 0.08, Re = 15000,
                                 sponge over the last 3H) with the reference's stone wall
-  box_c0()                      160 x 80 x 40, box wall H = 10, thickness 6 at x = 40,
+  box_wall_flow()               160 x 80 x 40, box wall H = 10, thickness 6 at x = 40,
                                 inlet 0.05 (z/H)^(1/7) capped at 0.06, nu = 2e-4, Cs = 0.16
   porous_wall(H, porosity, ...) 16H x 8H x 4H, wall at x = 4H, thickness 0.6H, rounded-box
                                 stones opened to a void fraction, gusting inlet, nu = 2e-4
@@ -19,7 +19,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import geometry
-from .inflow import baseline_profile, gust_table, make_gust, reference_setup
+from .inflow import power_law_profile, gust_table, make_gust, reference_setup
 from .lattice import flag_from_mask
 
 
@@ -61,9 +61,9 @@ def reference(H=40, porosity=0.2, seed=7, Re=15000.0, U0=0.08, Cs=0.16, n_steps=
                     dict(phi=phi, X0=s.X0, T=s.T, t_unit=s.t_unit, zeta=s.zeta))
 
 
-def box_c0(n_steps=2000, ybc="freeslip", sponge=True, nx=160, ny=80, nz=40, H=10, X0=40, T=6, nu0=2e-4):
+def box_wall_flow(n_steps=2000, ybc="freeslip", sponge=True, nx=160, ny=80, nz=40, H=10, X0=40, T=6, nu0=2e-4):
     wall = geometry.box_wall(nx, ny, nz, X0, T, H)
-    uin = baseline_profile(nz, H)
+    uin = power_law_profile(nz, H)
     return Workload(f"box wall H={H} {nx}x{ny}x{nz}", flag_from_mask(wall), uin, H, 0.05, nu0, 0.16,
                     nx - 3 * H if sponge else None, 0.05, ybc, None, n_steps, 0, dict(X0=X0, T=T))
 
@@ -73,7 +73,7 @@ def porous_wall(H=40, porosity=0.2, seed=0, gust_seed=None, n_steps=20000, nu0=2
     nx, ny, nz = dims[0] * H, dims[1] * H, dims[2] * H
     X0, T = 4 * H, max(2, int(round(0.6 * H)))
     wall, info = geometry.rounded_box_wall(nx, ny, nz, X0, T, H, porosity, seed)
-    uin = baseline_profile(nz, H)
+    uin = power_law_profile(nz, H)
     g = make_gust(seed if gust_seed is None else gust_seed)
     info.update(X0=X0, T=T, gust=dict(a=g.a.tolist(), f=g.f.tolist(), phi=g.phi.tolist(), amp=g.amp))
     gt = gust_table(g, n_steps) if gust_time_scale == 1.0 else \

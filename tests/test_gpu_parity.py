@@ -14,7 +14,7 @@ import torch
 
 from oracle import c_oracle as co
 from stonewall3d import geometry, workloads
-from stonewall3d.inflow import baseline_profile, gust_table, make_gust
+from stonewall3d.inflow import power_law_profile, gust_table, make_gust
 from stonewall3d.lattice import flag_from_mask
 from stonewall3d.lbm3d import Simulation
 
@@ -28,7 +28,7 @@ def small_case(nx=48, ny=13, nz=18, H=8, seed=3, X0=12, lone=True):
     if lone:
         wall[30, 4, 3] = True                            # a lone stone
     flag = flag_from_mask(wall)
-    uin = baseline_profile(nz, H)
+    uin = power_law_profile(nz, H)
     return flag, uin
 
 
@@ -133,7 +133,7 @@ def test_deterministic_and_ring_independent(gpu):
 def test_box_wall_1000_steps(gpu, ybc):
     """The 160x80x40 box-wall workload, 1000 steps: populations within 1e-5 relative of
     the oracle (bit-identical expected) and mean Cd within 0.5 %."""
-    w = workloads.box_c0(n_steps=1000, ybc=ybc)
+    w = workloads.box_wall_flow(n_steps=1000, ybc=ybc)
     flag, uin = w.flag, w.uin
     sim = Simulation(flag, uin, **w.sim_kwargs())
     sim.init()
