@@ -54,7 +54,7 @@ __device__ __forceinline__ StepArgs instance_args(const sw_instance& I, int k) {
 }
 
 template <class S, class TL, bool BUF>
-__global__ __launch_bounds__(64 * TL::WPB, TL::OCC) void k_batch_step(const void* __restrict__ table, int k) {
+__global__ __launch_bounds__(64 * TL::WPB, (TL::PAIR && sizeof(S) == 2) ? 6 : TL::OCC) void k_batch_step(const void* __restrict__ table, int k) {
     const TableHeader* h = static_cast<const TableHeader*>(table);
     const sw_instance* e = table_entries(table);
     const int n = __builtin_amdgcn_readfirstlane(h->n), total = __builtin_amdgcn_readfirstlane(h->total_items);
@@ -121,11 +121,11 @@ static void launch_batch_v(const void* table, int k, int blocks, hipStream_t s) 
     hipLaunchKernelGGL((k_batch_step<S, TL, BUF>), dim3(blocks), dim3(64 * TL::WPB), 0, s, table, k);
 }
 
-// variants as sw_set_step_variant; lattices with > 4 GiB population arrays use 64-bit
-// addressing and variant 0
+// variants as sw_set_step_variant; a batch holding a lattice with a population array of
+// 4 GiB or more uses 64-bit addressing with nontemporal stores (variant 1's tile)
 template <class S>
 static void launch_batch(const void* table, int k, int blocks, int variant, bool big, hipStream_t s) {
-    if (big) { launch_batch_v<S, TileV0, false>(table, k, blocks, s); return; }
+    if (big) { launch_batch_v<S, TileV1, false>(table, k, blocks, s); return; }
     // pairs need 16-bit storage and even nx for every lattice; the host checks nx
     if ((variant == 8 || variant == 9) && sizeof(S) != 2) variant = 1;
     switch (variant) {

@@ -232,7 +232,7 @@ __global__ void k_transpose(const E* __restrict__ src, E* __restrict__ dst, int 
 using namespace sw;
 
 // Step variants (the autotuner's choices, per dtype): the tile list of sw_step.hpp.
-// Lattices whose byte offsets exceed 32 bits always use variant 0 with 64-bit addressing.
+// Lattices whose byte offsets exceed 32 bits use 64-bit addressing with nontemporal stores.
 static int g_step_variant[4] = {7, 8, 8, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
 
 template <class S, class TL, bool BUF>
@@ -248,7 +248,8 @@ static bool fits_buffer(const StepArgs& a, size_t elem) {
 
 template <class S>
 static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
-    if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV0, false>(a, s); return; }
+    // 64-bit addressing (arrays of 4 GiB and more): plain sweep with nontemporal stores
+    if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV1, false>(a, s); return; }
     int v = g_step_variant[dtype];
     if ((v == 8 || v == 9) && sizeof(S) != 2) v = 1;    // pairs: 16-bit storage only (odd nx falls back in-kernel)
     switch (v) {

@@ -57,6 +57,11 @@ class Result:
         return self.finished - self.arrival
 
 
+def request_work(req):
+    """Cell updates a request asks for: 16H x 8H x 4H cells times its steps."""
+    return 512 * req.H ** 3 * req.steps
+
+
 def nat_name(dtype_code):
     return {v: k for k, v in nat.DTYPES.items()}[dtype_code]
 
@@ -72,10 +77,19 @@ def lattice_bytes(H, dtype, steps, ring):
 
 
 class Server:
-    """budget_bytes: HBM admitted across all resident lattices; K: steps per chunk."""
+    """budget_bytes: HBM admitted across all resident lattices; K: steps per chunk;
+    policy: "fifo" or "sjf" admission order; fill_cells: stop admitting once the resident
+    lattices hold this many cells (None: admit whatever the budget allows)."""
+
+    policy = "fifo"
+    fill_cells = None
 
     def __init__(self, budget_bytes=250e9, K=32, max_instances=64, graph=True, keep_force=False, noise=0.02,
-                 device="cuda", variants=None):
+                 device="cuda", variants=None, policy="fifo", fill_cells=None):
+        if policy not in ("fifo", "sjf"):
+            raise ValueError("policy must be 'fifo' or 'sjf'")
+        self.policy = policy
+        self.fill_cells = None if fill_cells is None else float(fill_cells)
         self.budget = float(budget_bytes)
         self.K = int(K)
         self.max_instances = int(max_instances)
@@ -140,19 +154,30 @@ class Server:
         return out
 
     def _admit(self, queue, t0):
-        """Admit queued requests in FIFO order while the budget and the batch slots allow
-        (a request that does not fit blocks those behind it).  Geometry is built on the
-        host and the lattice prepared on the side stream, so this overlaps chunks already
-        in flight; joining a batch waits for that batch's chunk to end."""
+        """Admit queued requests while the budget, the batch slots and the compute cap allow.
+
+        Order: "fifo" takes the oldest request and a request that does not fit blocks those
+        behind it; "sjf" takes the request with the least work (cells x steps) first.  With
+        `fill_cells` set, admission also stops once the resident lattices hold that many
+        cells (enough to keep the device busy): later requests wait in the queue rather
+        than slow every resident lattice down, which shortens the typical time-to-result
+        when the offered load exceeds capacity.  Geometry is built on the host and the
+        lattice prepared on the side stream, so this overlaps chunks already in flight;
+        joining a batch waits for that batch's chunk to end."""
         while queue:
-            req = queue[0]
+            if self.fill_cells and self._resident_cells() >= self.fill_cells:
+                return
+            k = 0
+            if self.policy == "sjf":
+                k = min(range(len(queue)), key=lambda j: (request_work(queue[j]), queue[j].arrival))
+            req = queue[k]
             need = lattice_bytes(req.H, req.dtype, req.steps, self.K)
             b = self._batch(req.dtype)
             if self.used + need > self.budget or len(b.members) >= self.max_instances:
                 if self.used == 0 and not b.members:
                     raise RuntimeError(f"request {req.rid} needs {need/1e9:.1f} GB > budget")
                 return
-            queue.pop(0)
+            queue.pop(k)
             with torch.cuda.stream(self.prep_stream):
                 lat = self.make_lattice(req)
             self.prep_stream.synchronize()
@@ -161,6 +186,9 @@ class Server:
             self.used += need
             with torch.cuda.stream(self.streams[req.dtype]):
                 b.add(lat)
+
+    def _resident_cells(self):
+        return sum(m.N for b in self.batches.values() for m in b.members)
 
     def serve(self, requests, realtime=True, log=None):
         """Run a finite trace; returns (results, report).
@@ -235,15 +263,20 @@ def main(argv=None):
     ap.add_argument("--K", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default tiles and 8 workgroups/CU")
+    ap.add_argument("--policy", default="sjf", choices=["fifo", "sjf"], help="admission order")
+    ap.add_argument("--fill-mcells", type=float, default=64.0,
+                    help="stop admitting once resident lattices hold this many million cells (0: budget only)")
     a = ap.parse_args(argv)
     reqs = trace(a.requests, a.rate, a.seed, a.steps_scale, (a.hmin, a.hmax))
-    srv = Server(budget_bytes=a.budget_gb * 1e9, K=a.K)
+    srv = Server(budget_bytes=a.budget_gb * 1e9, K=a.K, policy=a.policy,
+                 fill_cells=a.fill_mcells * 1e6 if a.fill_mcells > 0 else None)
     tuned = None if a.no_tune else srv.tune(sorted({r.dtype for r in reqs}))
     results, rep = srv.serve(reqs, log=lambda r: print(
         f"[serve] request {r.rid} done: {r.dtype} H={r.H} {r.steps} steps, latency {r.latency:.2f} s, "
         f"Cd {r.cd_mean:.4f}", file=sys.stderr, flush=True))
-    work = sum(512 * r.H ** 3 * r.steps for r in reqs)
-    rep.update(rate=a.rate, steps_scale=a.steps_scale, K=a.K, budget_gb=a.budget_gb,
+    work = sum(request_work(r) for r in reqs)
+    rep.update(rate=a.rate, steps_scale=a.steps_scale, K=a.K, budget_gb=a.budget_gb, policy=a.policy,
+               fill_mcells=a.fill_mcells,
                offered_mlups=work / max(reqs[-1].arrival, 1e-9) / 1e6,
                arrival_span_s=reqs[-1].arrival, tuned=tuned,
                by_dtype={d: dict(requests=sum(r.dtype == d for r in results),

@@ -82,6 +82,32 @@ def test_fp64_matches_oracle(gpu):
     assert np.allclose(sim.force_rows(), F, rtol=1e-12, atol=1e-16)
 
 
+def test_fp64_beyond_4gib_matches_oracle(gpu):
+    """A 320x320x280 FP64 lattice: 19 * N * 8 B = 4.36 GB per population array, past the
+    4 GiB reach of 32-bit buffer offsets, so the step takes the 64-bit-pointer kernel.
+    Two steps against the FP64 oracle within 1e-14, force rows within 1e-12."""
+    w = workloads.box_wall_flow(n_steps=2, nx=320, ny=320, nz=280, H=40, X0=80, T=12)
+    assert 19 * w.cells * 8 >= 1 << 32
+    sim = Simulation(w.flag, w.uin, dtype="f64", **w.sim_kwargs())
+    sim.init(noise=0.02, seed=5)
+    f = co.init(w.flag, w.uin, "f64", noise=0.02, seed=5)
+    yb = 1 if w.ybc == "freeslip" else 0
+    forces = []
+    for t in range(2):
+        f, F = co.step(f, w.flag, w.uin, w.nu0, w.cs, w.x_sponge, w.nu_sponge, ybc=yb, storage="f64")
+        forces.append(F)
+    sim.step(2)
+    g = sim.populations()
+    fl = (w.flag == 0)[None]
+    np.subtract(g, f, out=g)
+    np.abs(g, out=g)
+    g *= fl
+    rel = g.max() / np.abs(f).max()
+    print("fp64 > 4 GiB: max rel diff", rel)
+    assert rel < 1e-14
+    assert np.allclose(sim.force_rows(), np.array(forces), rtol=1e-12, atol=1e-16)
+
+
 def test_macro_and_accumulate_bit_identical(gpu):
     flag, uin = small_case()
     sim, f, _ = run_both(flag, uin, 5, "f32", 0)

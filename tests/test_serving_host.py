@@ -89,6 +89,49 @@ def test_admission_under_budget(monkeypatch):
     assert rep["cell_updates"] == sum(512 * 10 ** 3 * r.steps for r in reqs)
 
 
+def _fake_server(monkeypatch, budget, K, fb, **attrs):
+    srv = serving.Server.__new__(serving.Server)
+    srv.budget, srv.K, srv.max_instances, srv.keep_force, srv.used = budget, K, 64, False, 0.0
+    for k, v in attrs.items():
+        setattr(srv, k, v)
+    srv.batches, srv.streams = {"f32": fb}, {"f32": None}
+    monkeypatch.setattr(srv, "_batch", lambda dtype: fb, raising=False)
+    monkeypatch.setattr(srv, "make_lattice", lambda req: FakeLattice(req, K), raising=False)
+
+    class NoStream:
+        def synchronize(self):
+            pass
+
+    srv.prep_stream = NoStream()
+    monkeypatch.setattr(serving.torch.cuda, "stream", lambda s: _Null())
+    monkeypatch.setattr(serving, "nat_name", lambda code: "f32")
+    return srv
+
+
+def test_sjf_with_fill_cap_runs_shortest_first(monkeypatch):
+    """With a compute cap of one lattice, "sjf" admits the request with the least work
+    first; every request still completes and the cell-update count is exact."""
+    K = 10
+    steps = [90, 30, 60, 20, 50, 40]
+    reqs = [serving.Request(k, 0.0, 10, 0.2, k, s, "f32") for k, s in enumerate(steps)]
+    fb = FakeBatch(K)
+    srv = _fake_server(monkeypatch, 1e12, K, fb, policy="sjf", fill_cells=512 * 10 ** 3)
+    results, rep = srv.serve(reqs, realtime=False)
+    assert fb.max_resident == 1
+    assert [r.rid for r in results] == sorted(range(6), key=lambda k: steps[k])
+    assert rep["cell_updates"] == sum(512 * 10 ** 3 * s for s in steps)
+
+
+def test_fifo_keeps_arrival_order(monkeypatch):
+    K = 10
+    steps = [90, 30, 60]
+    reqs = [serving.Request(k, 0.0, 10, 0.2, k, s, "f32") for k, s in enumerate(steps)]
+    fb = FakeBatch(K)
+    srv = _fake_server(monkeypatch, 1e12, K, fb, policy="fifo", fill_cells=512 * 10 ** 3)
+    results, rep = srv.serve(reqs, realtime=False)
+    assert [r.rid for r in results] == [0, 1, 2]
+
+
 class _Null:
     def __enter__(self):
         return self
