@@ -95,9 +95,23 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
 
 /* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
  * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept
- * per dtype for the process: 0 = one 64-cell chunk per wavefront iteration, 1 = two,
- * 2 = one with the next chunk's loads issued first, 3 = 2 + nontemporal stores,
- * 4 = 0 + nontemporal stores, 5 = 1 + nontemporal stores. */
+ * per dtype for the process; every variant gives bit-identical populations.
+ *   0  one 64-cell chunk per wavefront iteration, one wavefront per row
+ *   1  0 + nontemporal stores
+ *   2  1 + nontemporal loads
+ *   3  1, each row split over 2 wavefronts
+ *   4  1, each row split over 4 wavefronts
+ *   5  2, each row split over 2 wavefronts
+ *   6  1 with two chunks per iteration
+ *   7  1 with the next chunk's loads issued first, rows over 2 wavefronts
+ *   8  pairs: a lane owns two adjacent cells, one 32-bit (16-bit storage) or 64-bit
+ *      (FP32) access per link, x-1 / x+1 neighbours by DPP lane shifts; NT stores
+ *   9  8, rows over 2 wavefronts      10  8, rows over 4 wavefronts
+ *   11 9 with cached stores
+ *   12 quads (16-bit storage): a lane owns four adjacent cells, one 64-bit access per
+ *      link; NT stores                 13  12, rows over 2 wavefronts
+ * FP64 storage maps 8..13 to 1, FP32 maps 12..13 to 9.  Arrays of 4 GiB or more use 64-bit addressing with
+ * variant 1's tile whatever the setting. */
 int sw_step_variants(void);
 int sw_set_step_variant(int dtype, int variant);
 int sw_get_step_variant(int dtype);

@@ -126,8 +126,9 @@ static void launch_batch_v(const void* table, int k, int blocks, hipStream_t s) 
 template <class S>
 static void launch_batch(const void* table, int k, int blocks, int variant, bool big, hipStream_t s) {
     if (big) { launch_batch_v<S, TileV1, false>(table, k, blocks, s); return; }
-    // pairs need 16-bit storage and even nx for every lattice; the host checks nx
-    if ((variant == 8 || variant == 9) && sizeof(S) != 2) variant = 1;
+    // pairs: 16- and 32-bit storage (rows of odd length fall back in-kernel)
+    if (variant >= 8 && sizeof(S) == 8) variant = 1;
+    if ((variant == 12 || variant == 13) && sizeof(S) == 4) variant = 9;
     switch (variant) {
 #define SW_CASE(id, TL) case id: launch_batch_v<S, TL, true>(table, k, blocks, s); break;
         SW_TILE_LIST(SW_CASE)
@@ -211,7 +212,7 @@ int sw_batch_download(const void* table, sw_instance* instances, int n, void* st
 int sw_batch_step(const void* table, int dtype, int k, int blocks, int variant, int big_offsets, void* stream) {
     SW_CHECK_ARG(table, "table must be non-null");
     SW_CHECK_ARG(dtype >= 0 && dtype <= 3, "dtype must be 0..3");
-    SW_CHECK_ARG(blocks > 0 && k >= 0 && variant >= 0 && variant < NUM_TILE_VARIANTS, "need blocks > 0, k >= 0, variant 0..7");
+    SW_CHECK_ARG(blocks > 0 && k >= 0 && variant >= 0 && variant < NUM_TILE_VARIANTS, "need blocks > 0, k >= 0 and a listed variant");
     int rc = batch_step_launch(table, dtype, k, blocks, variant, big_offsets != 0, (hipStream_t)stream);
     if (rc) return set_hip_error((hipError_t)rc, "sw_batch_step");
     return SW_OK;

@@ -80,9 +80,26 @@ template <> struct Codec<double> {
     static __device__ __forceinline__ double load(const double* p, size_t i, double, double) { return p[i]; }
     static __device__ __forceinline__ void store(double* p, size_t i, double v, double, double) { p[i] = v; }
 };
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bfloat2_t __attribute__((ext_vector_type(2)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
+
 template <> struct Codec<_Float16> {
     using T = float;
     using B = uint16_t;
+    // Two cells' values packed into one 32-bit word (low half = first cell), encoded exactly
+    // as encode(): the first goes to the low half (v_fma_mixlo_f16), the second to the
+    // high half (v_fma_mixhi_f16).
+    using Pending = half2_t;
+    static __device__ __forceinline__ Pending first(float v, float w, float scale) {
+        Pending p = {};
+        p.x = (_Float16)__builtin_fmaf(v, scale, -(w * scale));
+        return p;
+    }
+    static __device__ __forceinline__ uint32_t second(Pending p, float v, float w, float scale) {
+        p.y = (_Float16)__builtin_fmaf(v, scale, -(w * scale));
+        return __builtin_bit_cast(uint32_t, p);
+    }
     // w + h*2^-k with one rounding: h*2^-k is exact, so fma == mul-then-add bit for bit
     static __device__ __forceinline__ float decode(B b, float w, float inv_scale) {
         return __builtin_fmaf((float)__builtin_bit_cast(_Float16, b), inv_scale, w);
@@ -105,6 +122,16 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {   // round to nearest
 template <> struct Codec<bf16_t> {
     using T = float;
     using B = uint16_t;
+    // Two cells packed into one 32-bit word: both scaled values rounded by one
+    // v_cvt_pk_bf16_f32 (round to nearest even, as encode()).
+    using Pending = float;
+    static __device__ __forceinline__ Pending first(float v, float w, float scale) {
+        return __builtin_fmaf(v, scale, -(w * scale));
+    }
+    static __device__ __forceinline__ uint32_t second(Pending p, float v, float w, float scale) {
+        const float2_t f = {p, __builtin_fmaf(v, scale, -(w * scale))};
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bfloat2_t));
+    }
     static __device__ __forceinline__ float decode(B b, float w, float inv_scale) {
         return __builtin_fmaf(bf16_to_f32(b), inv_scale, w);
     }
@@ -153,13 +180,21 @@ template <class S> struct PopMem<S, true> {
     using B = typename Codec<S>::B;
     __amdgpu_buffer_rsrc_t rin, rout;
     __device__ __forceinline__ PopMem(const void* fin, void* fout) : rin(make_rsrc(fin)), rout(make_rsrc(fout)) {}
-    // 32-bit access (two adjacent 16-bit populations)
+    // 32-bit access (two adjacent 16-bit populations, or one 32-bit edge value)
     __device__ __forceinline__ uint32_t ld32(size_t row_bytes, uint32_t lane_bytes) const {
         return buf_load<uint32_t>(rin, lane_bytes, (uint32_t)row_bytes);
     }
     template <bool NT = false>
     __device__ __forceinline__ void st32(size_t row_bytes, uint32_t lane_bytes, uint32_t v) const {
         buf_store<uint32_t, NT ? 2 : 0>(v, rout, lane_bytes, (uint32_t)row_bytes);
+    }
+    // 64-bit access (two adjacent 32-bit populations)
+    __device__ __forceinline__ uint64_t ld64(size_t row_bytes, uint32_t lane_bytes) const {
+        return buf_load<uint64_t>(rin, lane_bytes, (uint32_t)row_bytes);
+    }
+    template <bool NT = false>
+    __device__ __forceinline__ void st64(size_t row_bytes, uint32_t lane_bytes, uint64_t v) const {
+        buf_store<uint64_t, NT ? 2 : 0>(v, rout, lane_bytes, (uint32_t)row_bytes);
     }
     template <bool NT = false>
     __device__ __forceinline__ B ld(size_t row_bytes, uint32_t lane_bytes) const {
@@ -182,6 +217,14 @@ template <class S> struct PopMem<S, false> {
     template <bool NT = false>
     __device__ __forceinline__ void st32(size_t row_bytes, uint32_t lane_bytes, uint32_t v) const {
         uint32_t* p = reinterpret_cast<uint32_t*>(out + row_bytes + lane_bytes);
+        if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
+    }
+    __device__ __forceinline__ uint64_t ld64(size_t row_bytes, uint32_t lane_bytes) const {
+        return *reinterpret_cast<const uint64_t*>(in + row_bytes + lane_bytes);
+    }
+    template <bool NT = false>
+    __device__ __forceinline__ void st64(size_t row_bytes, uint32_t lane_bytes, uint64_t v) const {
+        uint64_t* p = reinterpret_cast<uint64_t*>(out + row_bytes + lane_bytes);
         if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
     }
     template <bool NT = false>

@@ -233,7 +233,7 @@ using namespace sw;
 
 // Step variants (the autotuner's choices, per dtype): the tile list of sw_step.hpp.
 // Lattices whose byte offsets exceed 32 bits use 64-bit addressing with nontemporal stores.
-static int g_step_variant[4] = {7, 8, 8, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
+static int g_step_variant[4] = {9, 8, 8, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
 
 template <class S, class TL, bool BUF>
 static void launch_step_v(const StepArgs& a, hipStream_t s) {
@@ -251,7 +251,8 @@ static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
     // 64-bit addressing (arrays of 4 GiB and more): plain sweep with nontemporal stores
     if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV1, false>(a, s); return; }
     int v = g_step_variant[dtype];
-    if ((v == 8 || v == 9) && sizeof(S) != 2) v = 1;    // pairs: 16-bit storage only (odd nx falls back in-kernel)
+    if (v >= 8 && sizeof(S) == 8) v = 1;    // pairs: 16/32-bit storage (odd nx falls back in-kernel)
+    if ((v == 12 || v == 13) && sizeof(S) == 4) v = 9;   // quads: 16-bit storage only
     switch (v) {
 #define SW_CASE(id, TL) case id: launch_step_v<S, TL, true>(a, s); break;
         SW_TILE_LIST(SW_CASE)

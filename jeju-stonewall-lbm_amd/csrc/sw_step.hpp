@@ -129,12 +129,17 @@ __host__ __device__ inline uint32_t chunk_words(uint32_t nx) { return (nx + 64 *
 //   OCC : minimum wavefronts per SIMD the compiler must fit (0 = compiler's choice).  A cap
 //         of 8 forces 64 VGPRs and spills the 16-bit kernels to scratch (measured 100x
 //         slower), so the list below uses 0 throughout.
-//   PAIR: 16-bit storage only — a lane owns two adjacent cells, each link is one aligned
-//         32-bit load (x-1 / x+1 neighbours by whole-wavefront DPP shifts) and one 32-bit
-//         store; other storage forms ignore it
-template <int CPL_, bool PF_, bool NT_, bool NTL_ = false, int WPB_ = ROWS, int OCC_ = 0, bool PAIR_ = false>
+//   PAIR: 16- and 32-bit storage — a lane owns two adjacent cells, each link is one
+//         aligned 32-bit (16-bit forms) or 64-bit (FP32) load, x-1 / x+1 neighbours by
+//         whole-wavefront DPP shifts, and one store of the same width; FP64 ignores it
+//   QUAD: 16-bit storage — a lane owns four adjacent cells, one aligned 64-bit load and
+//         store per link; the host maps it to the pair tile for FP32 and to variant 1
+//         for FP64
+template <int CPL_, bool PF_, bool NT_, bool NTL_ = false, int WPB_ = ROWS, int OCC_ = 0, bool PAIR_ = false,
+          bool QUAD_ = false>
 struct Tile {
     static constexpr bool PAIR = PAIR_;
+    static constexpr bool QUAD = QUAD_;
     static constexpr int OCC = OCC_;
     static constexpr int CPL = CPL_;
     static constexpr bool PF = PF_;
@@ -153,11 +158,16 @@ using TileV4 = Tile<1, false, true, false, 16>;   // NT stores, rows split over 
 using TileV5 = Tile<1, false, true, true, 8>;     // NT stores and loads, rows over 2 wavefronts
 using TileV6 = Tile<2, false, true, false, 4>;    // 2 chunks per iteration, NT stores
 using TileV7 = Tile<1, true, true, false, 8>;     // prefetch, NT stores, rows over 2 wavefronts
-using TileV8 = Tile<1, false, true, false, 4, 0, true>;   // 16-bit pairs, NT stores
-using TileV9 = Tile<1, false, true, false, 8, 0, true>;   // 16-bit pairs, NT stores, rows over 2 wavefronts
-constexpr int NUM_TILE_VARIANTS = 10;
+using TileV8 = Tile<1, false, true, false, 4, 0, true>;   // pairs, NT stores
+using TileV9 = Tile<1, false, true, false, 8, 0, true>;   // pairs, NT stores, rows over 2 wavefronts
+using TileV10 = Tile<1, false, true, false, 16, 0, true>; // pairs, NT stores, rows over 4 wavefronts
+using TileV11 = Tile<1, false, false, false, 8, 0, true>; // pairs, cached stores, rows over 2 wavefronts
+using TileV12 = Tile<1, false, true, false, 4, 0, false, true>;  // 16-bit quads, NT stores
+using TileV13 = Tile<1, false, true, false, 8, 0, false, true>;  // 16-bit quads, NT stores, rows over 2 wavefronts
+constexpr int NUM_TILE_VARIANTS = 14;
 #define SW_TILE_LIST(X) X(0, TileV0) X(1, TileV1) X(2, TileV2) X(3, TileV3) X(4, TileV4) X(5, TileV5) \
-                        X(6, TileV6) X(7, TileV7) X(8, TileV8) X(9, TileV9)
+                        X(6, TileV6) X(7, TileV7) X(8, TileV8) X(9, TileV9) X(10, TileV10) X(11, TileV11) \
+                        X(12, TileV12) X(13, TileV13)
 
 // One (y, z) row of one lattice; adds this lane's FP64 wall-force sums to fx_acc/fz_acc.
 // BUF selects raw-buffer addressing (all byte offsets of a population array < 4 GiB).
@@ -289,53 +299,172 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     };
 
     bool inlet_done = false;
-    if constexpr (TL::PAIR && ES == 2) {
+    if constexpr (TL::QUAD && ES == 2) {
+      if (nx & 3u) {
+        plain_sweep();                    // quads need a row length divisible by 4
+      } else {
+        // 256-cell chunks; lane L owns cells xq = x0 + 4L .. xq + 3.  Each link is one aligned
+        // 64-bit load of four 16-bit values (512 B per wavefront instruction); the value
+        // that sits in the adjacent lane comes over by DPP and two 16-bit alignbits rebuild
+        // the shifted quad; the chunk-edge values come from one broadcast load per shifted
+        // link.  A chunk with a special 64-cell sub-chunk runs that sub-chunk on the
+        // general path and the others on the one-cell fast path.
+        const uint32_t nqc = (nx + 255) / 256;
+        const uint32_t qlen = (nqc + TL::NSEG - 1) / TL::NSEG;
+        const uint32_t qbeg = min(nqc, (uint32_t)seg * qlen);
+        const uint32_t qend = min(nqc, qbeg + qlen);
+#pragma clang loop unroll(disable)
+        for (uint32_t c = qbeg; c < qend; c++) {
+            const uint32_t x0 = c * 256;
+            bool special = false;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (4 * c + k < nch_row) special |= special_chunk(4 * c + k);
+            if (special) {
+#pragma unroll 1
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t sub = 4 * c + k;
+                    if (sub >= nch_row) break;
+                    const uint32_t x = sub * 64 + lane;
+                    if (special_chunk(sub)) {
+                        general_cell(x);
+                    } else if (x < nx && x != 0) {
+                        T fl[Q];
+                        load_fast(x, fl);
+                        collide_store(x, fl);
+                    }
+                }
+                continue;
+            }
+            const uint32_t xq = x0 + 4 * lane;
+            const uint32_t xl = min(xq, nx - 4);             // lanes past the row end read in-row
+            const uint32_t lb = xl * 2;
+            const uint32_t ledge = (x0 >= 2 ? x0 - 2 : 0) * 2;           // dword holding cells x0-2, x0-1
+            const uint32_t redge = (x0 + 256 < nx ? x0 + 256 : x0) * 2;   // dword holding cells x0+256, x0+257
+            uint32_t rl[Q], rh[Q];             // cells (xq, xq+1) and (xq+2, xq+3), 16 bits each
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
+                const uint64_t cur = mem.ld64(row, lb);
+                const uint32_t lo = (uint32_t)cur, hi = (uint32_t)(cur >> 32);
+                if constexpr (CX[i] == 0) {
+                    rl[i] = lo; rh[i] = hi;
+                } else if constexpr (CX[i] > 0) {                        // cells x-1
+                    const uint32_t p = wave_shr1(hi, mem.ld32(row, ledge));
+                    rl[i] = mid16(lo, p); rh[i] = mid16(hi, lo);
+                } else {                                                 // cells x+1
+                    uint32_t n = wave_shl1(lo, mem.ld32(row, redge));
+                    if (xl + 4 >= nx) n = hi >> 16;                      // outlet: cell nx-1 pulls itself
+                    rl[i] = mid16(hi, lo); rh[i] = mid16(n, hi);
+                }
+            });
+            uint32_t ol[Q], oh[Q];
+            {
+                typename C::Pending pend[Q];
+                T fl[Q];
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rl[I] & 0xFFFFu), wgt<T>(I), inv_scale); });
+                collide<T>(fl, xq, a, smag, [&](auto I, T v) { pend[I] = C::first(v, wgt<T>(I), scale); });
+                if (x0 == 0) {                                           // inlet cell x = 0 (:67-71)
+                    T u0 = (T)a.uin[z];
+                    if (a.gust) u0 = u0 * (T)a.gust[t];
+                    const T usq15 = T(1.5) * (u0 * u0);
+                    static_for<0, Q>([&](auto I) {
+                        constexpr int i = I;
+                        const typename C::Pending e =
+                            C::first(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
+                        pend[i] = (lane == 0) ? e : pend[i];
+                    });
+                    inlet_done = true;
+                }
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rl[I] >> 16), wgt<T>(I), inv_scale); });
+                collide<T>(fl, xq + 1, a, smag, [&](auto I, T v) { ol[I] = C::second(pend[I], v, wgt<T>(I), scale); });
+            }
+            {
+                typename C::Pending pend[Q];
+                T fl[Q];
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rh[I] & 0xFFFFu), wgt<T>(I), inv_scale); });
+                collide<T>(fl, xq + 2, a, smag, [&](auto I, T v) { pend[I] = C::first(v, wgt<T>(I), scale); });
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rh[I] >> 16), wgt<T>(I), inv_scale); });
+                collide<T>(fl, xq + 3, a, smag, [&](auto I, T v) { oh[I] = C::second(pend[I], v, wgt<T>(I), scale); });
+            }
+            if (xq < nx) {
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    mem.template st64<TL::NT>((size_t)i * NB + selfb, xq * 2, (uint64_t)ol[i] | ((uint64_t)oh[i] << 32));
+                });
+            }
+        }
+      }
+    } else if constexpr (TL::PAIR && ES == 2) {
       if (nx & 1u) {
         plain_sweep();                    // pairs need an even row length
       } else {
-        // 128-cell chunks; lane L owns cells x0 + 2L and x0 + 2L + 1 (nx is even)
+        // 128-cell chunks; lane L owns cells xa = x0 + 2L and xa + 1 (nx is even).  Each link
+        // is one aligned 32-bit load of two 16-bit values; the value that sits in the
+        // adjacent lane comes over by DPP and a 16-bit alignbit rebuilds the shifted pair;
+        // the chunk-edge values come from one broadcast load per shifted link.  (Issuing the
+        // next chunk's loads before this chunk's collisions needs 133 VGPRs, 3 wavefronts
+        // per SIMD, and measured 13 % slower.)
         const uint32_t npc = (nx + 127) / 128;
         const uint32_t plen = (npc + TL::NSEG - 1) / TL::NSEG;
         const uint32_t pbeg = min(npc, (uint32_t)seg * plen);
         const uint32_t pend = min(npc, pbeg + plen);
-#pragma clang loop unroll(disable)
-        for (uint32_t c = pbeg; c < pend; c++) {
-            const uint32_t x0 = c * 128;
-            const bool special = special_chunk(2 * c) || (2 * c + 1 < nch_row && special_chunk(2 * c + 1));
-            if (special) {
-                general_cell(x0 + lane);
-                general_cell(x0 + 64 + lane);
-                continue;
+        auto pair_special = [&](uint32_t c) {
+            return special_chunk(2 * c) || (2 * c + 1 < nch_row && special_chunk(2 * c + 1));
+        };
+        // a chunk with a special half: that half on the general path, the other on the
+        // one-cell fast path
+        auto special_pair_chunk = [&](uint32_t c) {
+#pragma unroll 1
+            for (uint32_t sub = 2 * c; sub < min(2 * c + 2, nch_row); sub++) {
+                const uint32_t x = sub * 64 + lane;
+                if (special_chunk(sub)) {
+                    general_cell(x);
+                } else if (x < nx && x != 0) {
+                    T fl[Q];
+                    load_fast(x, fl);
+                    collide_store(x, fl);
+                }
             }
-            const uint32_t xa = x0 + 2 * lane;
-            const uint32_t xl = min(xa, nx - 2);             // lanes past the row end read in-row
-            const uint32_t lb = xl * 2;
+        };
+        auto issue = [&](uint32_t c, uint32_t (&cur)[Q], uint32_t (&edge)[Q]) {
+            const uint32_t x0 = c * 128;
+            const uint32_t lb = min(x0 + 2 * lane, nx - 2) * 2;          // lanes past the row end read in-row
             const uint32_t ledge = (x0 >= 2 ? x0 - 2 : 0) * 2;            // dword holding cell x0-1
             const uint32_t redge = (x0 + 128 < nx ? x0 + 128 : x0) * 2;   // dword holding cell x0+128
-            uint32_t raw[Q];
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
                 const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
-                const uint32_t cur = mem.ld32(row, lb);
+                cur[i] = mem.ld32(row, lb);
+                if constexpr (CX[i] > 0) edge[i] = mem.ld32(row, ledge);
+                if constexpr (CX[i] < 0) edge[i] = mem.ld32(row, redge);
+            });
+        };
+        auto finish = [&](uint32_t c, const uint32_t (&cur)[Q], const uint32_t (&edge)[Q]) {
+            const uint32_t x0 = c * 128;
+            const uint32_t xa = x0 + 2 * lane;
+            const uint32_t xl = min(xa, nx - 2);
+            uint32_t raw[Q];
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
                 if constexpr (CX[i] == 0) {
-                    raw[i] = cur;
+                    raw[i] = cur[i];
                 } else if constexpr (CX[i] > 0) {                        // cells (x-1, x)
-                    raw[i] = mid16(cur, wave_shr1(cur, mem.ld32(row, ledge)));
+                    raw[i] = mid16(cur[i], wave_shr1(cur[i], edge[i]));
                 } else {                                                 // cells (x+1, x+2)
-                    uint32_t nxt = wave_shl1(cur, mem.ld32(row, redge));
-                    if (xl + 2 >= nx) nxt = cur >> 16;                  // outlet: cell nx-1 pulls itself
-                    raw[i] = mid16(nxt, cur);
+                    uint32_t nxt = wave_shl1(cur[i], edge[i]);
+                    if (xl + 2 >= nx) nxt = cur[i] >> 16;               // outlet: cell nx-1 pulls itself
+                    raw[i] = mid16(nxt, cur[i]);
                 }
             });
-            uint32_t packed[Q];
+            typename C::Pending pnd[Q];
+            uint32_t out[Q];
             {
                 T fl[Q];
                 static_for<0, Q>([&](auto I) {
                     fl[I] = C::decode((uint16_t)(raw[I] & 0xFFFFu), wgt<T>(I), inv_scale);
                 });
-                collide<T>(fl, xa, a, smag, [&](auto I, T v) {
-                    packed[I] = (uint32_t)C::encode(v, wgt<T>(I), scale);
-                });
+                collide<T>(fl, xa, a, smag, [&](auto I, T v) { pnd[I] = C::first(v, wgt<T>(I), scale); });
             }
             if (x0 == 0) {                                               // inlet cell x = 0 (:67-71)
                 T u0 = (T)a.uin[z];
@@ -343,9 +472,8 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                 const T usq15 = T(1.5) * (u0 * u0);
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    const uint32_t e =
-                        (uint32_t)C::encode(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
-                    packed[i] = (lane == 0) ? e : packed[i];
+                    const typename C::Pending e = C::first(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
+                    pnd[i] = (lane == 0) ? e : pnd[i];
                 });
                 inlet_done = true;
             }
@@ -354,14 +482,102 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                 static_for<0, Q>([&](auto I) {
                     fl[I] = C::decode((uint16_t)(raw[I] >> 16), wgt<T>(I), inv_scale);
                 });
-                collide<T>(fl, xa + 1, a, smag, [&](auto I, T v) {
-                    packed[I] |= (uint32_t)C::encode(v, wgt<T>(I), scale) << 16;
-                });
+                collide<T>(fl, xa + 1, a, smag, [&](auto I, T v) { out[I] = C::second(pnd[I], v, wgt<T>(I), scale); });
             }
             if (xa < nx) {
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    mem.template st32<TL::NT>((size_t)i * NB + selfb, xa * 2, packed[i]);
+                    mem.template st32<TL::NT>((size_t)i * NB + selfb, xa * 2, out[i]);
+                });
+            }
+        };
+#pragma clang loop unroll(disable)
+        for (uint32_t c = pbeg; c < pend; c++) {
+            if (pair_special(c)) {
+                special_pair_chunk(c);
+                continue;
+            }
+            uint32_t cur[Q], edge[Q];
+            issue(c, cur, edge);
+            finish(c, cur, edge);
+        }
+      }
+    } else if constexpr (TL::PAIR && ES == 4) {
+      if (nx & 1u) {
+        plain_sweep();                    // pairs need an even row length
+      } else {
+        // 128-cell chunks; lane L owns cells xa = x0 + 2L and xa + 1 (nx is even).  Each link
+        // is one aligned 64-bit load (512 B per wavefront instruction); the x-1 / x+1
+        // neighbours that sit in the adjacent lane move over by DPP, the two chunk-edge
+        // values come from one broadcast load per shifted link, and both cells' outputs go
+        // out as one 64-bit store per link.
+        const uint32_t npc = (nx + 127) / 128;
+        const uint32_t plen = (npc + TL::NSEG - 1) / TL::NSEG;
+        const uint32_t pbeg = min(npc, (uint32_t)seg * plen);
+        const uint32_t pend = min(npc, pbeg + plen);
+#pragma clang loop unroll(disable)
+        for (uint32_t c = pbeg; c < pend; c++) {
+            const uint32_t x0 = c * 128;
+            const bool special = special_chunk(2 * c) || (2 * c + 1 < nch_row && special_chunk(2 * c + 1));
+            if (special) {                   // the special half on the general path, the other one-cell fast
+#pragma unroll 1
+                for (uint32_t sub = 2 * c; sub < min(2 * c + 2, nch_row); sub++) {
+                    const uint32_t x = sub * 64 + lane;
+                    if (special_chunk(sub)) {
+                        general_cell(x);
+                    } else if (x < nx && x != 0) {
+                        T fl[Q];
+                        load_fast(x, fl);
+                        collide_store(x, fl);
+                    }
+                }
+                continue;
+            }
+            const uint32_t xa = x0 + 2 * lane;
+            const uint32_t xl = min(xa, nx - 2);             // lanes past the row end read in-row
+            const uint32_t lb = xl * 4;
+            const uint32_t ledge = (x0 >= 1 ? x0 - 1 : 0) * 4;            // cell x0-1 (x0 = 0: the inlet, rewritten)
+            const uint32_t redge = (x0 + 128 < nx ? x0 + 128 : x0) * 4;   // cell x0+128
+            T fa[Q], fb[Q];
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
+                const uint64_t cur = mem.ld64(row, lb);
+                const uint32_t lo = (uint32_t)cur, hi = (uint32_t)(cur >> 32);
+                uint32_t va, vb;
+                if constexpr (CX[i] == 0) {
+                    va = lo; vb = hi;
+                } else if constexpr (CX[i] > 0) {                        // cells (x-1, x)
+                    va = wave_shr1(hi, mem.ld32(row, ledge)); vb = lo;
+                } else {                                                 // cells (x+1, x+2)
+                    va = hi;
+                    vb = wave_shl1(lo, mem.ld32(row, redge));
+                    if (xl + 2 >= nx) vb = hi;                           // outlet: cell nx-1 pulls itself
+                }
+                fa[i] = C::decode(va, wgt<T>(i), inv_scale);
+                fb[i] = C::decode(vb, wgt<T>(i), inv_scale);
+            });
+            uint32_t oa[Q];
+            collide<T>(fa, xa, a, smag, [&](auto I, T v) { oa[I] = C::encode(v, wgt<T>(I), scale); });
+            if (x0 == 0) {                                               // inlet cell x = 0 (:67-71)
+                T u0 = (T)a.uin[z];
+                if (a.gust) u0 = u0 * (T)a.gust[t];
+                const T usq15 = T(1.5) * (u0 * u0);
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    const uint32_t e = C::encode(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
+                    oa[i] = (lane == 0) ? e : oa[i];
+                });
+                inlet_done = true;
+            }
+            uint64_t ob[Q];
+            collide<T>(fb, xa + 1, a, smag, [&](auto I, T v) {
+                ob[I] = (uint64_t)oa[I] | ((uint64_t)C::encode(v, wgt<T>(I), scale) << 32);
+            });
+            if (xa < nx) {
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    mem.template st64<TL::NT>((size_t)i * NB + selfb, xa * 4, ob[i]);
                 });
             }
         }

@@ -263,7 +263,7 @@ def main(argv=None):
         cells += w.cells
     tuned = None
     if not a.no_tune:
-        cand = [1, 3, 7] + ([8, 9] if a.dtype in ("f16", "bf16") and all(m.nx % 2 == 0 for m in b.members) else [])
+        cand = [1, 3, 7] + ([8, 9, 10] if a.dtype != "f64" else [])
         tuned = Autotuner().tune_batch(b, variants=cand, per_cu=(4, 8, 16, 32), chunks=2)
         print(f"[batch] tuned: variant {b.variant}, {b.blocks} workgroups", file=sys.stderr, flush=True)
         # restart every lattice from its initial state after tuning

@@ -144,7 +144,7 @@ class Server:
                 w = workloads.porous_wall(H=h, porosity=0.2, seed=h, n_steps=4 * self.K)
                 b.add(Lattice(w.flag, w.uin, dtype=d, ring=self.K, device=self.device, **w.sim_kwargs())
                       .init(noise=self.noise, seed=h))
-            cand = [1, 3, 7] + ([8, 9] if d in ("f16", "bf16") else [])
+            cand = [1, 3, 7] + ([8, 9, 10] if d != "f64" else [])
             tuner.tune_batch(b, variants=cand, per_cu=(4, 8, 16, 32), chunks=2)
             self.variants[d], self.blocks[d] = b.variant, b.blocks
             out[d] = (b.variant, b.blocks)

@@ -36,12 +36,13 @@ def lattice(w, dtype, seed, K):
     return Lattice(w.flag, w.uin, dtype=dtype, ring=K, **w.sim_kwargs()).init(noise=0.02, seed=seed)
 
 
+@pytest.mark.parametrize("variant", [VARIANT, 8])
 @pytest.mark.parametrize("dtype", ["f32", "f16"])
-def test_alone_equals_mix(gpu, dtype):
+def test_alone_equals_mix(gpu, dtype, variant):
     K = 8
     ws = [wl(*c) for c in CASES]
-    ref = [alone(w, dtype, VARIANT, c[2]) for w, c in zip(ws, CASES)]
-    b = Batch(dtype, max_instances=8, K=K, variant=VARIANT, graph=True)
+    ref = [alone(w, dtype, variant, c[2]) for w, c in zip(ws, CASES)]
+    b = Batch(dtype, max_instances=8, K=K, variant=variant, graph=True)
     lats = [lattice(w, dtype, c[2], K) for w, c in zip(ws, CASES)]
     for lat in lats:
         b.add(lat)

@@ -258,11 +258,13 @@ def test_cli_short_run(gpu, tmp_path):
     assert not os.path.isdir(os.path.join(out, "q.ckpt"))
 
 
-@pytest.mark.parametrize("variant", range(10))
-@pytest.mark.parametrize("storage,nx", [("f32", 200), ("f16", 200), ("bf16", 200), ("f16", 256), ("bf16", 77)])
+@pytest.mark.parametrize("variant", range(14))
+@pytest.mark.parametrize("storage,nx", [("f32", 200), ("f16", 200), ("bf16", 200), ("f16", 256), ("bf16", 77),
+                                        ("f32", 256), ("f32", 77), ("f16", 202)])
 def test_every_tile_variant_bit_identical(gpu, variant, storage, nx):
     """Every kernel variant the autotuner may pick is bit-identical to the oracle (row
-    lengths with a partial last chunk, whole 128-cell chunks, and an odd length)."""
+    lengths with a partial last chunk, whole 128-cell chunks, an odd length, and an even
+    length not divisible by 4, where the quad tiles fall back)."""
     from stonewall3d import _native as nat
     old = nat.lib().sw_get_step_variant(nat.DTYPES[storage])
     nat.call("sw_set_step_variant", nat.DTYPES[storage], variant)
@@ -276,11 +278,11 @@ def test_every_tile_variant_bit_identical(gpu, variant, storage, nx):
         nat.call("sw_set_step_variant", nat.DTYPES[storage], old)
 
 
-@pytest.mark.parametrize("variant", [8, 9])
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("variant", [8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "f32"])
 def test_pair_path_inlet_chunk(gpu, variant, storage):
-    """16-bit pair tiles when the first 128-cell chunk is free of solids: the inlet cell is
-    written inside the packed 32-bit store."""
+    """Pair and quad tiles when the first chunk is free of solids: the inlet cell is written
+    inside the packed 32-bit (16-bit pairs) or 64-bit (FP32 pairs, 16-bit quads) store."""
     from stonewall3d import _native as nat
     old = nat.lib().sw_get_step_variant(nat.DTYPES[storage])
     nat.call("sw_set_step_variant", nat.DTYPES[storage], variant)
