@@ -134,6 +134,14 @@ int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, in
 int sw_qcoarse(const float* U, const uint32_t* links, int nx, int ny, int nz, int x0, int ds, int qx, int qy, int qz,
                float* out, void* stream);
 
+/* Self-test of the step's division shortcut (no counterpart in the reference: the step
+ * replaces IEEE division by a Newton-refined hardware reciprocal and an FMA-corrected
+ * quotient, which must round exactly like IEEE division).  counts: device buffer of two
+ * uint64; after the stream completes, counts[0] = normal divisors over all 2^32 bit
+ * patterns whose reciprocal differs from IEEE 1/b, counts[1] = differing quotients among
+ * 2^32 hashed samples (|a| in [2^-40, 4), b in [2^-4, 4)).  Both must be 0. */
+int sw_selftest_division(unsigned long long* counts, void* stream);
+
 /* ---------------------------------------------------------------- batched form
  * No reference counterpart: the reference runs one lattice per process (run_all.sh).
  * Many independent lattices of one dtype advance together: one persistent launch per

@@ -285,6 +285,26 @@ __device__ __forceinline__ float div3(float x) {
 }
 __device__ __forceinline__ double div3(double x) { return x / 3.0; }
 
+// Correctly rounded reciprocal and quotient without the IEEE division sequence
+// (v_div_scale / v_div_fmas / v_div_fixup and mode switches).
+//   rcp_rn(b): one Newton step on the hardware reciprocal, y = y0 + y0 (1 - b y0) with two
+//     FMAs.  Equal to IEEE 1/b for every normal b whose reciprocal is normal (all 2^32 bit
+//     patterns checked on gfx950; sw_selftest_division repeats the check).
+//   div_rn(a, b, y), y = rcp_rn(b): q0 = a y, r = a - b q0 (exact by FMA), q = q0 + r y.
+//     With y = RN(1/b) this is RN(a/b) (Markstein's theorem) whenever nothing underflows;
+//     +0 / b gives +0.  The densities, momenta and stress norms divided here are normal or
+//     +0 in any finite flow.  The double forms keep the plain division.
+__device__ __forceinline__ float rcp_rn(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
+}
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+    const float q0 = a * y;
+    return __builtin_fmaf(__builtin_fmaf(-b, q0, a), y, q0);
+}
+__device__ __forceinline__ double rcp_rn(double b) { return 1.0 / b; }
+__device__ __forceinline__ double div_rn(double a, double b, double) { return a / b; }
+
 // PCG hash (Jarzynski & Olano, JCGT 2020) for the optional init noise
 __device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
     uint32_t state = v * 747796405u + 2891336453u;

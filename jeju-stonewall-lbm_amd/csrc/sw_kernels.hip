@@ -227,6 +227,39 @@ __global__ void k_transpose(const E* __restrict__ src, E* __restrict__ dst, int 
     }
 }
 
+// ------------------------------------------------------------------ division self-test
+// counts[0]: normal b (with a normal reciprocal) where rcp_rn(b) != IEEE 1/b, over all 2^32
+// bit patterns; counts[1]: hashed (a, b) samples, |a| in [2^-40, 4), b in [2^-4, 4), where
+// div_rn(a, b, rcp_rn(b)) != IEEE a/b.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+__global__ void k_zero_counts(unsigned long long* counts) {
+    if (threadIdx.x < 2) counts[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_selftest_division(unsigned long long* counts) {
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bad_rcp = 0, bad_div = 0;
+    for (uint64_t u = tid; u < (1ull << 32); u += stride) {
+        const uint32_t bits = (uint32_t)u;
+        const uint32_t ex = (bits >> 23) & 0xFFu;
+        if (ex != 0 && ex < 253) {
+            const float b = __uint_as_float(bits);
+            if (__float_as_uint(rcp_rn(b)) != __float_as_uint(1.0f / b)) bad_rcp++;
+        }
+        const uint32_t h1 = mix32(bits ^ 0x9e3779b9u), h2 = mix32(bits * 2654435761u + 0x7f4a7c15u);
+        const float a = __uint_as_float((h1 & 0x807FFFFFu) | ((87u + (h1 >> 24) % 42u) << 23));
+        const float b = __uint_as_float((h2 & 0x007FFFFFu) | ((123u + (h2 >> 28) % 6u) << 23));
+        if (__float_as_uint(div_rn(a, b, rcp_rn(b))) != __float_as_uint(a / b)) bad_div++;
+    }
+    if (bad_rcp) atomicAdd(&counts[0], bad_rcp);
+    if (bad_div) atomicAdd(&counts[1], bad_div);
+}
+
 }  // namespace sw
 
 using namespace sw;
@@ -402,6 +435,15 @@ int sw_qcoarse(const float* U, const uint32_t* links, int nx, int ny, int nz, in
     SW_CHECK_ARG(n < (1LL << 31), "frame too large");
     hipLaunchKernelGGL(k_qcoarse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, U, links, nx,
                        ny, nz, x0, ds, qx, qy, qz, out);
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_selftest_division(unsigned long long* counts, void* stream) {
+    SW_CHECK_ARG(counts, "counts must be a device buffer of two 64-bit counters");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, counts);
+    hipLaunchKernelGGL(k_selftest_division, dim3(16384), dim3(256), 0, s, counts);
     SW_HIP(hipGetLastError());
     return SW_OK;
 }

@@ -61,7 +61,8 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
         if constexpr (cx * cz != 0 && i != 11) pxz = cx * cz > 0 ? pxz + v : pxz - v;
         if constexpr (cy * cz != 0 && i != 15) pyz = cy * cz > 0 ? pyz + v : pyz - v;
     });
-    const T ux = mx / rho, uy = my / rho, uz = mz / rho;                        // :128-138
+    const T yr = rcp_rn(rho);                                                  // :128-138
+    const T ux = div_rn(mx, rho, yr), uy = div_rn(my, rho, yr), uz = div_rn(mz, rho, yr);
     const T third = div3(rho);
     const T nxx = (pxx - third) - (rho * ux) * ux;
     const T nyy = (pyy - third) - (rho * uy) * uy;
@@ -78,8 +79,8 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
     }
     const T tau0 = T(3) * nu + T(0.5);
     const T qn = sqrt(((nxx * nxx + nyy * nyy) + nzz * nzz) + T(2) * ((nxy * nxy + nxz * nxz) + nyz * nyz));
-    const T tau = T(0.5) * (tau0 + sqrt(tau0 * tau0 + (smag * qn) / rho));
-    const T keep = T(1) - T(1) / tau;
+    const T tau = T(0.5) * (tau0 + sqrt(tau0 * tau0 + div_rn(smag * qn, rho, yr)));
+    const T keep = T(1) - rcp_rn(tau);
     const T usq15 = T(1.5) * ((ux * ux + uy * uy) + uz * uz);
 
     // ---- regularized collision (:149-155)

@@ -79,6 +79,28 @@ def test_graph_replays_equal_eager(gpu):
         assert np.array_equal(x, y)
 
 
+def test_batch_with_lattice_beyond_4gib(gpu):
+    """A batch holding an FP64 lattice whose population array exceeds 4 GiB (320x320x280:
+    4.36 GB) runs on 64-bit addressing for every member; both members stay bit-identical
+    to their single-lattice runs."""
+    K = 2
+    big = workloads.box_wall_flow(n_steps=2, nx=320, ny=320, nz=280, H=40, X0=80, T=12)
+    small = wl(6, 0.2, 11, 2)
+    assert 19 * big.cells * 8 >= 1 << 32
+    ref = [alone(w, "f64", 1, seed) for w, seed in ((big, 5), (small, 11))]
+    b = Batch("f64", max_instances=4, K=K, graph=True)
+    lats = [lattice(big, "f64", 5, K), lattice(small, "f64", 11, K)]
+    for lat in lats:
+        b.add(lat)
+    assert b.big
+    b.run_to_completion()
+    for lat, (p, F) in zip(lats, ref):
+        assert lat.t == 2
+        assert np.array_equal(lat.populations(), p)
+        assert np.array_equal(lat.force_rows(), F)
+    b.close()
+
+
 def test_join_mid_run(gpu):
     K = 8
     ws = [wl(*c) for c in CASES]

@@ -71,6 +71,18 @@ def test_step_bit_identical(gpu, storage, ybc):
     assert np.allclose(Fg, F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
 
 
+def test_division_shortcut_rounds_like_ieee(gpu):
+    """The step divides with a Newton-refined hardware reciprocal and an FMA-corrected
+    quotient; the library's self-test compares them with IEEE division on the device: every
+    normal divisor's reciprocal (all 2^32 bit patterns) and 2^32 hashed quotients."""
+    from stonewall3d import _native as nat
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    nat.call("sw_selftest_division", counts.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    rcp_bad, div_bad = counts.tolist()
+    assert rcp_bad == 0 and div_bad == 0, (rcp_bad, div_bad)
+
+
 def test_fp64_matches_oracle(gpu):
     flag, uin = small_case()
     sim, f, F = run_both(flag, uin, 25, "f64", 1, gust=gust_table(make_gust(2), 25))
