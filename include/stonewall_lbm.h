@@ -81,7 +81,8 @@ int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx,
 /* One fused step f_in -> f_out.  Replaces k_step, lbm3d.py:57-155, launched at
  * lbm3d.py:359-361.
  *   uin        : [nz] inlet profile (lbm3d.py:274-275); gust: per-step inlet factor
- *                table indexed by t (NULL = no gust, the reference's steady inlet).
+ *                table indexed by the step index, so it must hold more entries than
+ *                that index (NULL = no gust, the reference's steady inlet).
  *   t, t_dev   : step index = t + *t_dev (t_dev may be NULL); selects the gust entry
  *                and the partials slot t % ring.
  *   nu0, cs, x_sponge, nu_sponge : as lbm3d.py:361 (nu_sponge = 0.05 there).
