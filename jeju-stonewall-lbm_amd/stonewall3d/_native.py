@@ -76,6 +76,9 @@ def lib(path=None):
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    # PyTorch-ROCm ships its own HIP runtime; it has to be the one loaded when the library
+    # resolves its HIP symbols, else the two runtimes each see no device
+    import torch  # noqa: F401
     if not os.path.exists(p):
         raise RuntimeError(f"{p} is not built; run `make -C {os.path.dirname(p)}` "
                            "(or __graft_entry__.build()); there is no fallback compute path")
