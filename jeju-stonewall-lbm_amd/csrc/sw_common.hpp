@@ -192,6 +192,8 @@ template <class S> struct PopMem<S, true> {
     __device__ __forceinline__ uint64_t ld64(size_t row_bytes, uint32_t lane_bytes) const {
         return buf_load<uint64_t>(rin, lane_bytes, (uint32_t)row_bytes);
     }
+    // per-lane byte offset of any population (all below 4 GiB here)
+    __device__ __forceinline__ B ld_at(size_t bytes) const { return buf_load<B>(rin, (uint32_t)bytes, 0u); }
     template <bool NT = false>
     __device__ __forceinline__ void st64(size_t row_bytes, uint32_t lane_bytes, uint64_t v) const {
         buf_store<uint64_t, NT ? 2 : 0>(v, rout, lane_bytes, (uint32_t)row_bytes);
@@ -222,6 +224,7 @@ template <class S> struct PopMem<S, false> {
     __device__ __forceinline__ uint64_t ld64(size_t row_bytes, uint32_t lane_bytes) const {
         return *reinterpret_cast<const uint64_t*>(in + row_bytes + lane_bytes);
     }
+    __device__ __forceinline__ B ld_at(size_t bytes) const { return *reinterpret_cast<const B*>(in + bytes); }
     template <bool NT = false>
     __device__ __forceinline__ void st64(size_t row_bytes, uint32_t lane_bytes, uint64_t v) const {
         uint64_t* p = reinterpret_cast<uint64_t*>(out + row_bytes + lane_bytes);

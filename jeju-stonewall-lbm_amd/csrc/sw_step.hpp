@@ -232,39 +232,44 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             mem.template st<TL::NT>((size_t)i * NB + selfb, xo, C::encode(v, wgt<T>(i), scale));
         });
     };
-    // general path: per-link bounce-back and wall force (:77-101)
+    // general path: per-link bounce-back and wall force (:77-101).  Each lane picks the
+    // address of every link (streamed source, or its own opposite population on a
+    // bounce-back link) and all 19 loads are issued before the first is used.  Lanes past
+    // the row end, at the inlet or on a solid cell load in-row values and store nothing.
     auto general_cell = [&](uint32_t x) {
-        if (!(x < nx && x != 0)) return;
-        const uint32_t mm = lrow[x];
-        if (mm & SOLID_BIT) return;                                                  // :65-66
-        const uint32_t xo = x * ES;
-        const uint32_t xpo = min(x + 1, nx - 1) * ES;
+        const bool in_row = x < nx && x != 0;
+        const uint32_t xs = in_row ? x : 1u;                                         // nx >= 2
+        const uint32_t mm = lrow[xs];
+        const bool live = in_row && !(mm & SOLID_BIT);                              // :65-66
+        const size_t xo = (size_t)xs * ES;
+        const size_t xpo = (size_t)min(xs + 1, nx - 1) * ES;
+        typename C::B raw[Q];
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            size_t off;
+            if constexpr (i == 0) off = selfb + xo;
+            else off = ((mm >> i) & 1u) ? (size_t)OPP[i] * NB + selfb + xo : srow[i] + ((CX[i] < 0) ? xpo : xo);
+            raw[i] = mem.ld_at(off);
+        });
+        if (!live) return;
         T fl[Q];
         T fxw = T(0), fzw = T(0);
         bool hit = false;
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             constexpr T w = wgt<T>(i);
-            if constexpr (i == 0) {
-                fl[0] = C::decode(mem.ld(selfb, xo), w, inv_scale);
-            } else {
-                T val;
-                if ((mm >> i) & 1u) {                                                // half-way bounce-back
-                    val = C::decode(mem.ld((size_t)OPP[i] * NB + selfb, xo), w, inv_scale);
-                    bool wall = true;
-                    if constexpr (GROUND_SLOT[i] >= 0) wall = !((mm >> (GROUND_BIT0 + GROUND_SLOT[i])) & 1u);
-                    if (wall) {                                                      // momentum exchange
-                        const T fr2 = T(2) * (val - w);
-                        if constexpr (CX[i] > 0) fxw = fxw - fr2;
-                        if constexpr (CX[i] < 0) fxw = fxw + fr2;
-                        if constexpr (CZ[i] > 0) fzw = fzw - fr2;
-                        if constexpr (CZ[i] < 0) fzw = fzw + fr2;
-                        hit = true;
-                    }
-                } else {
-                    val = C::decode(mem.ld(srow[i], (CX[i] < 0) ? xpo : xo), w, inv_scale);
+            fl[i] = C::decode(raw[i], w, inv_scale);
+            if constexpr (i != 0) {
+                bool wall = (mm >> i) & 1u;                                          // half-way bounce-back
+                if constexpr (GROUND_SLOT[i] >= 0) wall = wall && !((mm >> (GROUND_BIT0 + GROUND_SLOT[i])) & 1u);
+                if (wall) {                                                          // momentum exchange
+                    const T fr2 = T(2) * (fl[i] - w);
+                    if constexpr (CX[i] > 0) fxw = fxw - fr2;
+                    if constexpr (CX[i] < 0) fxw = fxw + fr2;
+                    if constexpr (CZ[i] > 0) fzw = fzw - fr2;
+                    if constexpr (CZ[i] < 0) fzw = fzw + fr2;
+                    hit = true;
                 }
-                fl[i] = val;
             }
         });
         if (hit) { fx_acc += (double)fxw; fz_acc += (double)fzw; }
