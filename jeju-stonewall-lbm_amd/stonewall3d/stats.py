@@ -49,8 +49,15 @@ def blur3(a, n=1):
 
 def recirculation_area(mean_ux, wall, H, X0, UH, ny):
     """mean_ux, wall: full-resolution (nx, ny, nz) time-mean u_x and solid mask."""
-    ux = np.asarray(mean_ux)[::2, ::2, ::2].astype(np.float16).astype(np.float32)
-    solid = np.asarray(wall)[::2, ::2, ::2]
+    return recirculation_area_ds2(np.asarray(mean_ux)[::2, ::2, ::2].astype(np.float16), np.asarray(wall)[::2, ::2, ::2],
+                                  H, X0, UH, ny)
+
+
+def recirculation_area_ds2(ux_ds2, wall_ds2, H, X0, UH, ny):
+    """The same from the stored half-resolution fields (the .npz keys mean3d_ds2[0] and
+    wall_ds2, as export3d.py reads them)."""
+    ux = np.asarray(ux_ds2).astype(np.float32)
+    solid = np.asarray(wall_ds2)
     i0, i1 = X0 // 2, min(ux.shape[0], (X0 + 10 * H) // 2)
     k1 = min(ux.shape[2], int(2 * H) // 2)
     sub = np.where(solid[i0:i1, :, :k1], 0.05, ux[i0:i1, :, :k1]) / UH
