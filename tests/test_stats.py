@@ -51,3 +51,15 @@ def test_recirculation_area_slab():
     # solids count as forward flow
     wall[:, :, :h] = True
     assert stats.recirculation_area(ux, wall, H, X0, UH, ny) == 0.0
+
+
+def test_published_cases_fixture():
+    """The validation driver reads the reference's published statistics for its three
+    H = 40 cases from the golden fixture."""
+    import os
+    from stonewall3d.validate import published_cases
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_results.json")
+    refs, info = published_cases(path)
+    assert set(refs) == {"solid", "p20", "p40"} and int(info["H"]) == 40
+    assert refs["p20"]["porosity_target"] == 0.2
+    assert abs(refs["solid"]["Cd"] - 1.4029) < 1e-4 and abs(refs["p40"]["area"] - 0.5776) < 1e-3
