@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--shift", default="inlet", choices=["inlet", "weights"],
+                    help="16-bit forms: store f_i - feq_i(1, uin[z]) (inlet) or f_i - w_i (weights)")
     ap.add_argument("--workload", default="porous_wall", choices=["porous_wall", "reference"],
                     help="porous_wall: 16H x 8H x 4H rounded-box stone wall with a gusting inlet; reference: "
                          "lbm3d.py's own default case (20H x 4H x (8H+1), its stone wall, seed 7, periodic y)")
@@ -117,7 +119,7 @@ def main():
                 f"free-slip top/sides")
     nx, ny, nz = w.shape
     N = nx * ny * nz
-    sim = Simulation(w.flag, w.uin, dtype=a.dtype, **w.sim_kwargs())
+    sim = Simulation(w.flag, w.uin, dtype=a.dtype, shift=a.shift, **w.sim_kwargs())
     sim.init(noise=0.02, seed=0)
     sim.step(a.warmup)
     torch.cuda.synchronize()
@@ -163,7 +165,8 @@ def main():
         "data": "synthetic (seeded rounded-box stone wall, seeded gust)" if a.workload == "porous_wall"
                 else "synthetic (the reference's stone-wall generator, seed 7)",
         "config": {"workload": desc.format(f"{nx}x{ny}x{nz}"),
-                   "cells": N, "fluid_cells": fluid, "bytes_per_update": bpu},
+                   "cells": N, "fluid_cells": fluid, "bytes_per_update": bpu,
+                   "storage_offset": sim.shift_kind if a.dtype in ("f16", "bf16") else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step", "kernel_avg_ms": round(kern_ms, 4)},

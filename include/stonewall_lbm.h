@@ -19,8 +19,14 @@
  *       f[i*N + (z*ny + y)*nx + x],  N = nx*ny*nz,  i = 0..18 in the reference's
  *     direction order (lbm3d.py:20-24).  sw_to_ref_layout / sw_from_ref_layout convert
  *     to and from the reference's (19, nx, ny, nz) z-contiguous arrays.
- *   - dtype: SW_F32 (float), SW_F16 / SW_BF16 (16-bit storage of (f_i - w_i) * scale,
- *     FP32 arithmetic; scale a power of two), SW_F64 (double storage and arithmetic).
+ *   - dtype: SW_F32 (float), SW_F16 / SW_BF16 (16-bit storage, FP32 arithmetic; see
+ *     below), SW_F64 (double storage and arithmetic).
+ *   - 16-bit storage: the weight-normalised population phi_i = f_i / w_i is stored as
+ *     (phi_i - sigma) * scale, scale a power of two in [2^-20, 2^20], sigma = sigma_c(z) of
+ *     the direction's c_x class c (0, +1, -1) at the cell's height, from a `shift` table
+ *     [nz][4] float (device; sw_shift_table) or 1 when `shift` is NULL.  The collision runs
+ *     on phi in FP32 (moments as FMA chains with the weights; the rest as the reference).
+ *     The same table must be passed to every call on one population state.
  *   - A lattice holds at most 113,025,455 cells.
  */
 #ifndef STONEWALL_LBM_H
@@ -34,6 +40,7 @@ extern "C" {
 
 enum sw_dtype { SW_F32 = 0, SW_F16 = 1, SW_BF16 = 2, SW_F64 = 3 };
 enum sw_ybc { SW_Y_PERIODIC = 0, SW_Y_FREESLIP = 1 };
+enum sw_shift { SW_SHIFT_WEIGHTS = 0, SW_SHIFT_INLET = 1 };
 enum sw_flag { SW_FLUID = 0, SW_WALL = 1, SW_GROUND = 2 };           /* lbm3d.py:30 */
 enum sw_error {
     SW_OK = 0,
@@ -76,7 +83,7 @@ int sw_prepare_links(const uint8_t* flag, uint32_t* links, int nx, int ny, int n
  * launched at lbm3d.py:340.  Fluid cells get u = (uin[z](1+noise r1), uin[z] noise r2,
  * uin[z] noise r3) with r in [-1,1) from a PCG hash of (seed, cell); solid cells u = 0. */
 int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx, int ny, int nz,
-            float noise, unsigned int seed, float scale, void* stream);
+            float noise, unsigned int seed, float scale, const float* shift, void* stream);
 
 /* One fused step f_in -> f_out.  Replaces k_step, lbm3d.py:57-155, launched at
  * lbm3d.py:359-361.
@@ -86,13 +93,14 @@ int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx,
  *   t, t_dev   : step index = t + *t_dev (t_dev may be NULL); selects the gust entry
  *                and the partials slot t % ring.
  *   nu0, cs, x_sponge, nu_sponge : as lbm3d.py:361 (nu_sponge = 0.05 there).
+ *   scale, shift : the 16-bit storage's scale and offset table (ignored for F32 / F64).
  *   partials   : FP64 force-partials ring, ring * sw_partials_per_step(ny, nz) doubles,
  *                or NULL to skip the force.  Reduce with sw_force_reduce.
  * Solid cells of f_out are not written (as the reference, lbm3d.py:65-66). */
 int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
             const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
-            int x_sponge, double nu_sponge, int ybc, float scale, double* partials, int ring,
-            void* stream);
+            int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
+            int ring, void* stream);
 
 /* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
  * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept
@@ -126,8 +134,18 @@ int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, 
 /* (rho, ux, uy, uz) per cell into U[4][N] and/or Uacc[4][N] += that (either may be NULL).
  * Replaces k_macro + k_accum, lbm3d.py:158-188, launched at lbm3d.py:367-369 and :387.
  * Solid cells give (1, 0, 0, 0). */
-int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale, float* U,
-             float* Uacc, void* stream);
+int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale,
+             const float* shift, float* U, float* Uacc, void* stream);
+
+/* Offset table of the 16-bit storage forms (no reference counterpart: the reference
+ * stores FP32).  shift: [nz][4] float (device) output, (sigma_0, sigma_+1, sigma_-1, 0) per
+ * height, sigma_c = feq_i(1, (U, 0, 0)) / w_i for a direction with c_x = c, evaluated in the
+ * operand order of lbm3d.py:35-36 in FP32.
+ *   SW_SHIFT_WEIGHTS: U = 0, sigma = 1: the stored value is (f_i - w_i) / w_i.
+ *   SW_SHIFT_INLET  : U = uin[z], the inlet velocity at that height.  Where the flow is
+ *                     near the inlet profile the stored value is ~10-50x smaller than
+ *                     (f_i - w_i) / w_i, and so is its rounding error. */
+int sw_shift_table(const float* uin, int nz, int kind, float* shift, void* stream);
 
 /* Max-pooled positive Q-criterion frame of U (from sw_macro) over ds^3 blocks of the crop
  * starting at (x0, 0, 0): out[i][j][k], (qx, qy, qz).  Replaces k_qcoarse + q_at,
@@ -157,6 +175,7 @@ typedef struct sw_instance {
     const float* gust;        /* [t_end] inlet factor per step, or NULL */
     double* partials;         /* ring * sw_partials_per_step(ny, nz) doubles, or NULL */
     double* force;            /* [t_end][nz][2] force record, or NULL */
+    const float* shift;       /* 16-bit storage offsets [nz][4] (sw_shift_table), or NULL */
     double nu0, cs, nu_sponge;
     float scale;
     int nx, ny, nz;

@@ -11,7 +11,7 @@
 
 namespace sw {
 
-constexpr int SW_VERSION = 100;   // 1.0.0
+constexpr int SW_VERSION = 110;   // 1.1.0
 // library error codes (SW_ERR_*) come from stonewall_lbm.h; hipError_t values (< 1000)
 // are passed through unchanged
 
@@ -49,8 +49,8 @@ inline bool is_pow2_float(float s) {
 
 inline int check_dtype(int dtype, float scale) {
     if (dtype < 0 || dtype > 3) return set_error(SW_ERR_INVALID, "dtype must be 0 (f32), 1 (f16), 2 (bf16) or 3 (f64)");
-    if ((dtype == 1 || dtype == 2) && !is_pow2_float(scale))
-        return set_error(SW_ERR_INVALID, "scale of a 16-bit storage form must be a positive power of two");
+    if ((dtype == 1 || dtype == 2) && !(is_pow2_float(scale) && scale >= 0x1p-20f && scale <= 0x1p20f))
+        return set_error(SW_ERR_INVALID, "scale of a 16-bit storage form must be a power of two in [2^-20, 2^20]");
     return SW_OK;
 }
 

@@ -47,6 +47,7 @@ __device__ __forceinline__ StepArgs instance_args(const sw_instance& I, int k) {
     a.f_in = odd ? I.f1 : I.f0;
     a.f_out = odd ? I.f0 : I.f1;
     a.links = I.links; a.uin = I.uin; a.gust = I.gust; a.partials = I.partials; a.t_dev = nullptr;
+    a.shift = (I.dtype == DT_F16 || I.dtype == DT_BF16) ? I.shift : nullptr;
     a.ring = I.ring; a.nx = I.nx; a.ny = I.ny; a.nz = I.nz; a.nyb = (I.ny + ROWS - 1) / ROWS;
     a.x_sponge = I.x_sponge; a.ybc = I.ybc; a.nu0 = I.nu0; a.cs = I.cs; a.nu_sponge = I.nu_sponge;
     a.scale = I.scale; a.inv_scale = 1.0f / I.scale;

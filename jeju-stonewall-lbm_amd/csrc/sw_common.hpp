@@ -56,12 +56,18 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 // ------------------------------------------------------------- storage codecs
 enum Dtype { DT_F32 = 0, DT_F16 = 1, DT_BF16 = 2, DT_F64 = 3 };
+// Offsets sigma of the 16-bit storage forms (per-height table built by sw_shift_table): the
+// normalised equilibrium of the inlet velocity at each height, or 1 (U = 0).
+enum ShiftKind { SHIFT_WEIGHTS = 0, SHIFT_INLET = 1 };
 
 struct bf16_t { uint16_t bits; };
 
 // Storage policy: S is the element type in HBM, T the arithmetic type, B the raw bits.
-// The 16-bit forms store (f_i - w_i) * scale; scale is a power of two so the scaling is
-// exact.  decode/encode convert between raw bits and arithmetic values.
+// The 16-bit forms store (v - s) * scale, v the weight-normalised population f_i / w_i and
+// s its offset sigma (a per-height value of the direction's c_x class); scale is a power
+// of two so the scaling is exact.  decode(b, s, 1/scale) and load/store take the offset s;
+// encode / first / second take ns = -s * scale (neg_scaled), which the step keeps in scalar
+// registers.
 template <class S> struct Codec;
 
 template <> struct Codec<float> {
@@ -91,22 +97,23 @@ template <> struct Codec<_Float16> {
     // as encode(): the first goes to the low half (v_fma_mixlo_f16), the second to the
     // high half (v_fma_mixhi_f16).
     using Pending = half2_t;
-    static __device__ __forceinline__ Pending first(float v, float w, float scale) {
+    static __device__ __forceinline__ Pending first(float v, float ns, float scale) {
         Pending p = {};
-        p.x = (_Float16)__builtin_fmaf(v, scale, -(w * scale));
+        p.x = (_Float16)__builtin_fmaf(v, scale, ns);
         return p;
     }
-    static __device__ __forceinline__ uint32_t second(Pending p, float v, float w, float scale) {
-        p.y = (_Float16)__builtin_fmaf(v, scale, -(w * scale));
+    static __device__ __forceinline__ uint32_t second(Pending p, float v, float ns, float scale) {
+        p.y = (_Float16)__builtin_fmaf(v, scale, ns);
         return __builtin_bit_cast(uint32_t, p);
     }
     // w + h*2^-k with one rounding: h*2^-k is exact, so fma == mul-then-add bit for bit
     static __device__ __forceinline__ float decode(B b, float w, float inv_scale) {
         return __builtin_fmaf((float)__builtin_bit_cast(_Float16, b), inv_scale, w);
     }
-    // RN(v - w) * 2^k == RN(v*2^k - w*2^k): scaling by a power of two commutes with rounding
-    static __device__ __forceinline__ B encode(float v, float w, float scale) {
-        return __builtin_bit_cast(B, (_Float16)__builtin_fmaf(v, scale, -(w * scale)));
+    // RN(v - w) * 2^k == RN(v*2^k - w*2^k): scaling by a power of two commutes with rounding.
+    // ns = -w*2^k (neg_scaled).
+    static __device__ __forceinline__ B encode(float v, float ns, float scale) {
+        return __builtin_bit_cast(B, (_Float16)__builtin_fmaf(v, scale, ns));
     }
     static __device__ __forceinline__ float load(const _Float16* p, size_t i, float w, float inv_scale) {
         return w + (float)p[i] * inv_scale;
@@ -125,18 +132,18 @@ template <> struct Codec<bf16_t> {
     // Two cells packed into one 32-bit word: both scaled values rounded by one
     // v_cvt_pk_bf16_f32 (round to nearest even, as encode()).
     using Pending = float;
-    static __device__ __forceinline__ Pending first(float v, float w, float scale) {
-        return __builtin_fmaf(v, scale, -(w * scale));
+    static __device__ __forceinline__ Pending first(float v, float ns, float scale) {
+        return __builtin_fmaf(v, scale, ns);
     }
-    static __device__ __forceinline__ uint32_t second(Pending p, float v, float w, float scale) {
-        const float2_t f = {p, __builtin_fmaf(v, scale, -(w * scale))};
+    static __device__ __forceinline__ uint32_t second(Pending p, float v, float ns, float scale) {
+        const float2_t f = {p, __builtin_fmaf(v, scale, ns)};
         return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bfloat2_t));
     }
     static __device__ __forceinline__ float decode(B b, float w, float inv_scale) {
         return __builtin_fmaf(bf16_to_f32(b), inv_scale, w);
     }
-    static __device__ __forceinline__ B encode(float v, float w, float scale) {
-        return f32_to_bf16(__builtin_fmaf(v, scale, -(w * scale)));
+    static __device__ __forceinline__ B encode(float v, float ns, float scale) {
+        return f32_to_bf16(__builtin_fmaf(v, scale, ns));
     }
     static __device__ __forceinline__ float load(const bf16_t* p, size_t i, float w, float inv_scale) {
         return w + bf16_to_f32(p[i].bits) * inv_scale;
@@ -145,6 +152,16 @@ template <> struct Codec<bf16_t> {
         p[i].bits = f32_to_bf16((v - w) * scale);
     }
 };
+
+// -s * 2^k by integer operations on the float's bits: exact for a normal s whose scaled
+// value stays normal (the storage offsets are near 1 and 2^-20 <= 2^k <= 2^20), and done by
+// the scalar ALU when s is wave-uniform, so the result stays in a scalar register.
+// s = 0 -> -0.
+__host__ __device__ __forceinline__ float neg_scaled(float s, int k) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, s);
+    const uint32_t r = (b & 0x7fffffffu) ? (b + ((uint32_t)k << 23)) : b;
+    return __builtin_bit_cast(float, r ^ 0x80000000u);
+}
 
 // ------------------------------------------------------------- memory access
 // Raw-buffer access: one descriptor per array, a wave-uniform 32-bit byte offset (row
@@ -264,6 +281,32 @@ __device__ __forceinline__ T cdotu(T ux, T uy, T uz) {
         if (have) s = s + t; else s = t;
     }
     return s;
+}
+
+// c_x class of a direction for the 16-bit storage offsets: 0 (c_x = 0), 1 (+1), 2 (-1)
+__host__ __device__ constexpr int xclass(int cx) { return cx == 0 ? 0 : (cx > 0 ? 1 : 2); }
+
+template <int I, class T>
+__device__ __forceinline__ T feq(T rho, T ux, T uy, T uz, T usq15);
+
+// feq_i(1, u) / w_i = ((1 + cu) + (0.5 cu) cu) - usq15 (lbm3d.py:35-36 order without the
+// weight): the equilibrium of the 16-bit forms, which hold phi_i = f_i / w_i
+template <int I, class T>
+__device__ __forceinline__ T feq_norm(T ux, T uy, T uz, T usq15) {
+    if constexpr (I == 0) {
+        return T(1) - usq15;
+    } else {
+        const T cu = T(3) * cdotu<I>(ux, uy, uz);
+        return ((T(1) + cu) + (T(0.5) * cu) * cu) - usq15;
+    }
+}
+
+// value written at an inlet cell (lbm3d.py:67-71): feq_i(1, (u0, 0, 0)), or its normalised
+// form for the 16-bit storage (NORM)
+template <bool NORM, int I, class T>
+__device__ __forceinline__ T inlet_value(T u0, T usq15) {
+    if constexpr (NORM) return feq_norm<I, T>(u0, T(0), T(0), usq15);
+    else return feq<I, T>(T(1), u0, T(0), T(0), usq15);
 }
 
 template <int I, class T>

@@ -73,7 +73,7 @@ __global__ void k_chunk_flags(uint32_t* __restrict__ links, int nx, int ny, int 
 // ------------------------------------------------------------------ init (lbm3d.py:39-54)
 template <class S>
 __global__ void k_init(S* __restrict__ f, const uint32_t* __restrict__ links, const float* __restrict__ uin,
-                       int nx, int ny, int nz, float noise, uint32_t seed, float scale) {
+                       int nx, int ny, int nz, float noise, uint32_t seed, float scale, const float* __restrict__ shift) {
     using C = Codec<S>;
     using T = typename C::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -94,14 +94,19 @@ __global__ void k_init(S* __restrict__ f, const uint32_t* __restrict__ links, co
     const T usq15 = T(1.5) * ((ux * ux + uy * uy) + uz * uz);
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
-        C::store(f, (size_t)i * N + self, feq<i, T>(T(1), ux, uy, uz, usq15), wgt<T>(i), (T)scale);
+        if constexpr (sizeof(S) == 2) {         // 16-bit forms: phi_i = feq_i / w_i, offset sigma
+            const T sg = shift ? (T)shift[z * 4 + xclass(CX[i])] : T(1);
+            C::store(f, (size_t)i * N + self, feq_norm<i, T>(ux, uy, uz, usq15), sg, (T)scale);
+        } else {
+            C::store(f, (size_t)i * N + self, feq<i, T>(T(1), ux, uy, uz, usq15), wgt<T>(i), (T)scale);
+        }
     });
 }
 
 // ------------------------------------------------------------------ macro + accumulate (lbm3d.py:158-188)
 template <class S>
 __global__ void k_macro(const S* __restrict__ f, const uint32_t* __restrict__ links, int nx, int ny, int nz,
-                        float inv_scale, float* __restrict__ U, float* __restrict__ acc) {
+                        float inv_scale, const float* __restrict__ shift, float* __restrict__ U, float* __restrict__ acc) {
     using C = Codec<S>;
     using T = typename C::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -116,11 +121,21 @@ __global__ void k_macro(const S* __restrict__ f, const uint32_t* __restrict__ li
         T rho = T(0), mx = T(0), my = T(0), mz = T(0);
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            const T v = C::load(f, (size_t)i * N + self, wgt<T>(i), (T)inv_scale);
-            rho = rho + v;
-            if constexpr (CX[i] != 0) mx = CX[i] > 0 ? mx + v : mx - v;
-            if constexpr (CY[i] != 0) my = CY[i] > 0 ? my + v : my - v;
-            if constexpr (CZ[i] != 0) mz = CZ[i] > 0 ? mz + v : mz - v;
+            if constexpr (sizeof(S) == 2) {     // phi_i = f_i / w_i: the step's FMA chains
+                const T sg = shift ? (T)shift[z * 4 + xclass(CX[i])] : T(1);
+                const T v = C::load(f, (size_t)i * N + self, sg, (T)inv_scale);
+                constexpr T w = wgt<T>(i);
+                rho = fma(w, v, rho);
+                if constexpr (CX[i] != 0) mx = fma(CX[i] > 0 ? w : -w, v, mx);
+                if constexpr (CY[i] != 0) my = fma(CY[i] > 0 ? w : -w, v, my);
+                if constexpr (CZ[i] != 0) mz = fma(CZ[i] > 0 ? w : -w, v, mz);
+            } else {
+                const T v = C::load(f, (size_t)i * N + self, wgt<T>(i), (T)inv_scale);
+                rho = rho + v;
+                if constexpr (CX[i] != 0) mx = CX[i] > 0 ? mx + v : mx - v;
+                if constexpr (CY[i] != 0) my = CY[i] > 0 ? my + v : my - v;
+                if constexpr (CZ[i] != 0) mz = CZ[i] > 0 ? mz + v : mz - v;
+            }
         });
         u[0] = (float)rho; u[1] = (float)(mx / rho); u[2] = (float)(my / rho); u[3] = (float)(mz / rho);
     }
@@ -171,6 +186,22 @@ __global__ void k_qcoarse(const float* __restrict__ U, const uint32_t* __restric
                 best = q > best ? q : best;
             }
     out[((size_t)i * qy + j) * qz + k] = best;
+}
+
+// ------------------------------------------------------------------ storage offsets of the 16-bit forms
+// shift[z][c] = sigma_c(z) = feq_i(1, (U, 0, 0)) / w_i for a direction of c_x class c (0, +1,
+// -1), U = uin[z] (kind SHIFT_INLET) or 0 (SHIFT_WEIGHTS: sigma = 1); shift[z][3] = 0.  The
+// 16-bit forms store (f_i / w_i - sigma) * scale, small wherever the flow is close to the
+// inlet profile.  Evaluated as the step evaluates the equilibrium (lbm3d.py:35-36 order).
+__global__ void k_shift_table(const float* __restrict__ uin, int nz, int kind, float* __restrict__ shift) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nz) return;
+    const float u0 = kind == SHIFT_INLET ? uin[z] : 0.0f;
+    const float usq15 = 1.5f * (u0 * u0);
+    shift[z * 4 + 0] = feq_norm<3, float>(u0, 0.0f, 0.0f, usq15);     // c = (0, 1, 0): c_x = 0
+    shift[z * 4 + 1] = feq_norm<1, float>(u0, 0.0f, 0.0f, usq15);     // c_x = +1
+    shift[z * 4 + 2] = feq_norm<2, float>(u0, 0.0f, 0.0f, usq15);     // c_x = -1
+    shift[z * 4 + 3] = 0.0f;
 }
 
 // ------------------------------------------------------------------ force rows
@@ -340,17 +371,17 @@ int sw_prepare_links(const uint8_t* flag, uint32_t* links, int nx, int ny, int n
 }
 
 int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx, int ny, int nz, float noise,
-            unsigned int seed, float scale, void* stream) {
+            unsigned int seed, float scale, const float* shift, void* stream) {
     SW_CHECK_ARG(f && links && uin, "f, links and uin must be non-null");
     SW_CHECK_RC(check_dims(nx, ny, nz));
     SW_CHECK_RC(check_dtype(dtype, scale));
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((nx + 255) / 256, ny, nz), blk(256);
     switch (dtype) {
-        case DT_F32: hipLaunchKernelGGL(k_init<float>, grid, blk, 0, s, (float*)f, links, uin, nx, ny, nz, noise, seed, 1.0f); break;
-        case DT_F64: hipLaunchKernelGGL(k_init<double>, grid, blk, 0, s, (double*)f, links, uin, nx, ny, nz, noise, seed, 1.0f); break;
-        case DT_F16: hipLaunchKernelGGL(k_init<_Float16>, grid, blk, 0, s, (_Float16*)f, links, uin, nx, ny, nz, noise, seed, scale); break;
-        case DT_BF16: hipLaunchKernelGGL(k_init<bf16_t>, grid, blk, 0, s, (bf16_t*)f, links, uin, nx, ny, nz, noise, seed, scale); break;
+        case DT_F32: hipLaunchKernelGGL(k_init<float>, grid, blk, 0, s, (float*)f, links, uin, nx, ny, nz, noise, seed, 1.0f, nullptr); break;
+        case DT_F64: hipLaunchKernelGGL(k_init<double>, grid, blk, 0, s, (double*)f, links, uin, nx, ny, nz, noise, seed, 1.0f, nullptr); break;
+        case DT_F16: hipLaunchKernelGGL(k_init<_Float16>, grid, blk, 0, s, (_Float16*)f, links, uin, nx, ny, nz, noise, seed, scale, shift); break;
+        case DT_BF16: hipLaunchKernelGGL(k_init<bf16_t>, grid, blk, 0, s, (bf16_t*)f, links, uin, nx, ny, nz, noise, seed, scale, shift); break;
     }
     SW_HIP(hipGetLastError());
     return SW_OK;
@@ -358,7 +389,7 @@ int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx,
 
 int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
             int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
-            int ybc, float scale, double* partials, int ring, void* stream) {
+            int ybc, float scale, const float* shift, double* partials, int ring, void* stream) {
     SW_CHECK_ARG(f_in && f_out && links && uin, "f_in, f_out, links and uin must be non-null");
     SW_CHECK_ARG(f_in != f_out, "f_in and f_out must be different buffers (pull scheme)");
     SW_CHECK_RC(check_dims(nx, ny, nz));
@@ -369,6 +400,7 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
     StepArgs a{};
     a.f_in = f_in; a.f_out = f_out; a.links = links; a.uin = uin; a.gust = gust;
     a.partials = partials; a.t_dev = t_dev; a.t = t; a.ring = ring > 0 ? ring : 1;
+    a.shift = (dtype == DT_F16 || dtype == DT_BF16) ? shift : nullptr;
     a.nx = nx; a.ny = ny; a.nz = nz; a.nyb = (ny + ROWS - 1) / ROWS;
     a.x_sponge = x_sponge; a.ybc = ybc; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
     a.scale = scale; a.inv_scale = 1.0f / scale;
@@ -406,8 +438,8 @@ int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, 
     return SW_OK;
 }
 
-int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale, float* U,
-             float* Uacc, void* stream) {
+int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale,
+             const float* shift, float* U, float* Uacc, void* stream) {
     SW_CHECK_ARG(f && links && (U || Uacc), "f, links and one of U / Uacc must be non-null");
     SW_CHECK_RC(check_dims(nx, ny, nz));
     SW_CHECK_RC(check_dtype(dtype, scale));
@@ -415,11 +447,20 @@ int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, in
     dim3 grid((nx + 255) / 256, ny, nz), blk(256);
     const float inv = 1.0f / scale;
     switch (dtype) {
-        case DT_F32: hipLaunchKernelGGL(k_macro<float>, grid, blk, 0, s, (const float*)f, links, nx, ny, nz, 1.0f, U, Uacc); break;
-        case DT_F64: hipLaunchKernelGGL(k_macro<double>, grid, blk, 0, s, (const double*)f, links, nx, ny, nz, 1.0f, U, Uacc); break;
-        case DT_F16: hipLaunchKernelGGL(k_macro<_Float16>, grid, blk, 0, s, (const _Float16*)f, links, nx, ny, nz, inv, U, Uacc); break;
-        case DT_BF16: hipLaunchKernelGGL(k_macro<bf16_t>, grid, blk, 0, s, (const bf16_t*)f, links, nx, ny, nz, inv, U, Uacc); break;
+        case DT_F32: hipLaunchKernelGGL(k_macro<float>, grid, blk, 0, s, (const float*)f, links, nx, ny, nz, 1.0f, nullptr, U, Uacc); break;
+        case DT_F64: hipLaunchKernelGGL(k_macro<double>, grid, blk, 0, s, (const double*)f, links, nx, ny, nz, 1.0f, nullptr, U, Uacc); break;
+        case DT_F16: hipLaunchKernelGGL(k_macro<_Float16>, grid, blk, 0, s, (const _Float16*)f, links, nx, ny, nz, inv, shift, U, Uacc); break;
+        case DT_BF16: hipLaunchKernelGGL(k_macro<bf16_t>, grid, blk, 0, s, (const bf16_t*)f, links, nx, ny, nz, inv, shift, U, Uacc); break;
     }
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_shift_table(const float* uin, int nz, int kind, float* shift, void* stream) {
+    SW_CHECK_ARG(shift && (uin || kind == SHIFT_WEIGHTS), "shift (and uin for the inlet kind) must be non-null");
+    SW_CHECK_ARG(nz > 0, "nz must be positive");
+    SW_CHECK_ARG(kind == SHIFT_WEIGHTS || kind == SHIFT_INLET, "kind must be 0 (weights) or 1 (inlet equilibrium)");
+    hipLaunchKernelGGL(k_shift_table, dim3((nz + 63) / 64), dim3(64), 0, (hipStream_t)stream, uin, nz, kind, shift);
     SW_HIP(hipGetLastError());
     return SW_OK;
 }

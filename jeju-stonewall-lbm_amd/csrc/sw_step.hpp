@@ -29,6 +29,7 @@ struct StepArgs {
     const float* gust;       // [t] gust factor per step, or null (factor 1)
     double* partials;        // [ring][nz][nyb][2], or null
     const int* t_dev;        // optional device step counter added to t
+    const float* shift;      // 16-bit storage: [nz][19] stored offset s_i(z), or null (s_i = w_i)
     int t;
     int ring;
     int nx, ny, nz, nyb;
@@ -40,27 +41,51 @@ struct StepArgs {
 
 // Collision of one cell from its 19 streamed populations fl[] (lbm3d.py:103-155); the
 // results are stored through st(i, value).
-template <class T, class Store>
+//   NORM = false: fl are the populations f_i; the reference's FP32 operation order.
+//   NORM = true : fl are the weight-normalised populations phi_i = f_i / w_i of the 16-bit
+//                 storage forms, and so are the results.  The moments are FMA chains in i
+//                 order, rho = sum w_i phi_i, m_a = sum (c_ia w_i) phi_i, Pi_ab = sum
+//                 (c_ia c_ib w_i) phi_i, each from +0; the stress, viscosity and relaxation
+//                 are those of NORM = false; phi_out = fma(rho, feq_i / w_i, (keep * 4.5) * qq).
+template <bool NORM, class T, class Store>
 __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const StepArgs& a, T smag, Store&& st) {
-    // ---- moments (:103-127), sequential in i; zero-coefficient terms dropped
-    T rho = fl[0], mx, my, mz, pxx, pyy, pzz, pxy, pxz, pyz;
-    // first contributions: 0 + v == v for the (never -0) populations
-    mx = fl[1]; pxx = fl[1];
-    my = fl[3]; pyy = fl[3];
-    mz = fl[5]; pzz = fl[5];
-    pxy = fl[7]; pxz = fl[11]; pyz = fl[15];
-    static_for<1, Q>([&](auto I) {
-        constexpr int i = I;
-        constexpr int cx = CX[i], cy = CY[i], cz = CZ[i];
-        const T v = fl[i];
-        rho = rho + v;
-        if constexpr (cx != 0 && i != 1) { mx = cx > 0 ? mx + v : mx - v; pxx = pxx + v; }
-        if constexpr (cy != 0 && i != 3) { my = cy > 0 ? my + v : my - v; pyy = pyy + v; }
-        if constexpr (cz != 0 && i != 5) { mz = cz > 0 ? mz + v : mz - v; pzz = pzz + v; }
-        if constexpr (cx * cy != 0 && i != 7) pxy = cx * cy > 0 ? pxy + v : pxy - v;
-        if constexpr (cx * cz != 0 && i != 11) pxz = cx * cz > 0 ? pxz + v : pxz - v;
-        if constexpr (cy * cz != 0 && i != 15) pyz = cy * cz > 0 ? pyz + v : pyz - v;
-    });
+    T rho, mx, my, mz, pxx, pyy, pzz, pxy, pxz, pyz;
+    if constexpr (NORM) {
+        rho = mx = my = mz = pxx = pyy = pzz = pxy = pxz = pyz = T(0);
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = CX[i], cy = CY[i], cz = CZ[i];
+            constexpr T w = wgt<T>(i);
+            const T v = fl[i];
+            rho = fma(w, v, rho);
+            if constexpr (cx != 0) { mx = fma(cx > 0 ? w : -w, v, mx); pxx = fma(w, v, pxx); }
+            if constexpr (cy != 0) { my = fma(cy > 0 ? w : -w, v, my); pyy = fma(w, v, pyy); }
+            if constexpr (cz != 0) { mz = fma(cz > 0 ? w : -w, v, mz); pzz = fma(w, v, pzz); }
+            if constexpr (cx * cy != 0) pxy = fma(cx * cy > 0 ? w : -w, v, pxy);
+            if constexpr (cx * cz != 0) pxz = fma(cx * cz > 0 ? w : -w, v, pxz);
+            if constexpr (cy * cz != 0) pyz = fma(cy * cz > 0 ? w : -w, v, pyz);
+        });
+    } else {
+        // ---- moments (:103-127), sequential in i; zero-coefficient terms dropped
+        rho = fl[0];
+        // first contributions: 0 + v == v for the (never -0) populations
+        mx = fl[1]; pxx = fl[1];
+        my = fl[3]; pyy = fl[3];
+        mz = fl[5]; pzz = fl[5];
+        pxy = fl[7]; pxz = fl[11]; pyz = fl[15];
+        static_for<1, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = CX[i], cy = CY[i], cz = CZ[i];
+            const T v = fl[i];
+            rho = rho + v;
+            if constexpr (cx != 0 && i != 1) { mx = cx > 0 ? mx + v : mx - v; pxx = pxx + v; }
+            if constexpr (cy != 0 && i != 3) { my = cy > 0 ? my + v : my - v; pyy = pyy + v; }
+            if constexpr (cz != 0 && i != 5) { mz = cz > 0 ? mz + v : mz - v; pzz = pzz + v; }
+            if constexpr (cx * cy != 0 && i != 7) pxy = cx * cy > 0 ? pxy + v : pxy - v;
+            if constexpr (cx * cz != 0 && i != 11) pxz = cx * cz > 0 ? pxz + v : pxz - v;
+            if constexpr (cy * cz != 0 && i != 15) pyz = cy * cz > 0 ? pyz + v : pyz - v;
+        });
+    }
     const T yr = rcp_rn(rho);                                                  // :128-138
     const T ux = div_rn(mx, rho, yr), uy = div_rn(my, rho, yr), uz = div_rn(mz, rho, yr);
     const T third = div3(rho);
@@ -91,10 +116,12 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
     constexpr T third_c = T(1) / T(3);
     constexpr T A1 = T(1) - third_c;      // (c*c - 1/3) for |c| = 1
     constexpr T A0 = T(0) - third_c;      // (c*c - 1/3) for c = 0
+    const T K = keep * T(4.5);            // NORM: (keep * 4.5) * qq is the normalised neq
     {
         constexpr T w = wgt<T>(0);
         const T qq = (A0 * nxx + A0 * nyy) + A0 * nzz;
-        st(std::integral_constant<int, 0>{}, feq<0, T>(rho, ux, uy, uz, usq15) + ((keep * w) * T(4.5)) * qq);
+        if constexpr (NORM) st(std::integral_constant<int, 0>{}, fma(rho, T(1) - usq15, K * qq));
+        else st(std::integral_constant<int, 0>{}, feq<0, T>(rho, ux, uy, uz, usq15) + ((keep * w) * T(4.5)) * qq);
     }
     static_for<0, 9>([&](auto P) {
         constexpr int i = 2 * P + 1, j = i + 1;
@@ -105,12 +132,18 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
         if constexpr (cx * cy != 0) qq = qq + T(2) * (cx * cy > 0 ? nxy : -nxy);
         if constexpr (cx * cz != 0) qq = qq + T(2) * (cx * cz > 0 ? nxz : -nxz);
         if constexpr (cy * cz != 0) qq = qq + T(2) * (cy * cz > 0 ? nyz : -nyz);
-        const T neq = ((keep * w) * T(4.5)) * qq;
         const T cu = T(3) * cdotu<i>(ux, uy, uz);
         const T h = (T(0.5) * cu) * cu;
-        const T wr = w * rho;
-        st(std::integral_constant<int, i>{}, wr * (((T(1) + cu) + h) - usq15) + neq);
-        st(std::integral_constant<int, j>{}, wr * (((T(1) + (-cu)) + h) - usq15) + neq);
+        if constexpr (NORM) {
+            const T kq = K * qq;
+            st(std::integral_constant<int, i>{}, fma(rho, ((T(1) + cu) + h) - usq15, kq));
+            st(std::integral_constant<int, j>{}, fma(rho, ((T(1) + (-cu)) + h) - usq15, kq));
+        } else {
+            const T neq = ((keep * w) * T(4.5)) * qq;
+            const T wr = w * rho;
+            st(std::integral_constant<int, i>{}, wr * (((T(1) + cu) + h) - usq15) + neq);
+            st(std::integral_constant<int, j>{}, wr * (((T(1) + (-cu)) + h) - usq15) + neq);
+        }
     });
 }
 
@@ -188,6 +221,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     const uint32_t selfrow = rowid * nx;
     const size_t selfb = (size_t)selfrow * ES;
     const T inv_scale = (T)a.inv_scale, scale = (T)a.scale;
+    const int kscale = (int)((__builtin_bit_cast(uint32_t, a.scale) >> 23) & 0xFFu) - 127;
     // (18*sqrt(2))*cs*cs evaluated left to right as lbm3d.py:146 does
     const T cs = (T)a.cs;
     const T smag = ((T(18) * (T)sqrt(T(2))) * cs) * cs;
@@ -196,6 +230,40 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     // rules, pre-shifted by -cx so that every load of a cell uses the lane offset x*ES
     // (cx = -1 loads use the outlet-clamped lane offset instead: lbm3d.py:82-83)
     size_t srow[Q];
+    // 16-bit forms: the stored value of direction i at height z is (phi_i - sigma_c(z)) * scale,
+    // phi_i = f_i / w_i, with sigma depending on the direction only through its c_x class
+    // (0, +1, -1).  Per row this needs sigma of the three classes at heights z-1, z, z+1
+    // (clamped to the lattice; the top row's c_z = -1 links mirror onto its own height) for
+    // decodes, and -sigma(z) * scale of the three classes for encodes: twelve wave-uniform
+    // values.  Mirror images of a direction share its c_x, so the y / z mirrors need no
+    // other offsets.  The FP32 / FP64 codecs ignore offsets: compile-time weights are
+    // passed instead.
+    constexpr bool NORM = (ES == 2);
+    T sg[3][3], en[3];
+    if constexpr (NORM) {
+        const int zl[3] = {z > 0 ? z - 1 : 0, z, z + 1 < (int)nz ? z + 1 : z};
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) sg[l][c] = a.shift ? (T)a.shift[zl[l] * 4 + c] : T(1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) en[c] = (T)neg_scaled((float)sg[1][c], kscale);
+    }
+    auto dsh = [&](auto I) -> T {                     // decode offset of link I's source
+        constexpr int i = I;
+        if constexpr (NORM) return sg[CZ[i] > 0 ? 0 : (CZ[i] < 0 ? 2 : 1)][xclass(CX[i])];
+        else return wgt<T>(i);
+    };
+    auto esh = [&](auto I) -> T {                     // encode constant of direction I
+        constexpr int i = I;
+        if constexpr (NORM) return en[xclass(CX[i])];
+        else return wgt<T>(i);
+    };
+    auto bsh = [&](auto I) -> T {                     // decode offset of the own opposite population
+        constexpr int i = I;
+        if constexpr (NORM) return sg[1][xclass(-CX[i])];
+        else return wgt<T>(OPP[i]);
+    };
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
         int zs = z - CZ[i], ys = y - CY[i], j = i;
@@ -222,14 +290,14 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         const uint32_t xpo = min(x + 1, nx - 1) * ES;
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            fl[i] = C::decode(mem.template ld<TL::NTL>(srow[i], (CX[i] < 0) ? xpo : xo), wgt<T>(i), inv_scale);
+            fl[i] = C::decode(mem.template ld<TL::NTL>(srow[i], (CX[i] < 0) ? xpo : xo), dsh(I), inv_scale);
         });
     };
     auto collide_store = [&](uint32_t x, const T (&fl)[Q]) {
         const uint32_t xo = x * ES;
-        collide<T>(fl, x, a, smag, [&](auto I, T v) {
+        collide<NORM, T>(fl, x, a, smag, [&](auto I, T v) {
             constexpr int i = I;
-            mem.template st<TL::NT>((size_t)i * NB + selfb, xo, C::encode(v, wgt<T>(i), scale));
+            mem.template st<TL::NT>((size_t)i * NB + selfb, xo, C::encode(v, esh(I), scale));
         });
     };
     // general path: per-link bounce-back and wall force (:77-101).  Each lane picks the
@@ -258,12 +326,14 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             constexpr T w = wgt<T>(i);
-            fl[i] = C::decode(raw[i], w, inv_scale);
+            if constexpr (i == 0) fl[i] = C::decode(raw[i], dsh(I), inv_scale);
+            else fl[i] = C::decode(raw[i], ((mm >> i) & 1u) ? bsh(I) : dsh(I), inv_scale);
             if constexpr (i != 0) {
                 bool wall = (mm >> i) & 1u;                                          // half-way bounce-back
                 if constexpr (GROUND_SLOT[i] >= 0) wall = wall && !((mm >> (GROUND_BIT0 + GROUND_SLOT[i])) & 1u);
                 if (wall) {                                                          // momentum exchange
-                    const T fr2 = T(2) * (fl[i] - w);
+                    // 2 (f_i - w_i); the 16-bit forms hold phi_i = f_i / w_i: (2 w_i) (phi_i - 1)
+                    const T fr2 = NORM ? (T(2) * w) * (fl[i] - T(1)) : T(2) * (fl[i] - w);
                     if constexpr (CX[i] > 0) fxw = fxw - fr2;
                     if constexpr (CX[i] < 0) fxw = fxw + fr2;
                     if constexpr (CZ[i] > 0) fzw = fzw - fr2;
@@ -368,8 +438,8 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             {
                 typename C::Pending pend[Q];
                 T fl[Q];
-                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rl[I] & 0xFFFFu), wgt<T>(I), inv_scale); });
-                collide<T>(fl, xq, a, smag, [&](auto I, T v) { pend[I] = C::first(v, wgt<T>(I), scale); });
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rl[I] & 0xFFFFu), dsh(I), inv_scale); });
+                collide<NORM, T>(fl, xq, a, smag, [&](auto I, T v) { pend[I] = C::first(v, esh(I), scale); });
                 if (x0 == 0) {                                           // inlet cell x = 0 (:67-71)
                     T u0 = (T)a.uin[z];
                     if (a.gust) u0 = u0 * (T)a.gust[t];
@@ -377,21 +447,21 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                     static_for<0, Q>([&](auto I) {
                         constexpr int i = I;
                         const typename C::Pending e =
-                            C::first(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
+                            C::first(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale);
                         pend[i] = (lane == 0) ? e : pend[i];
                     });
                     inlet_done = true;
                 }
-                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rl[I] >> 16), wgt<T>(I), inv_scale); });
-                collide<T>(fl, xq + 1, a, smag, [&](auto I, T v) { ol[I] = C::second(pend[I], v, wgt<T>(I), scale); });
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rl[I] >> 16), dsh(I), inv_scale); });
+                collide<NORM, T>(fl, xq + 1, a, smag, [&](auto I, T v) { ol[I] = C::second(pend[I], v, esh(I), scale); });
             }
             {
                 typename C::Pending pend[Q];
                 T fl[Q];
-                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rh[I] & 0xFFFFu), wgt<T>(I), inv_scale); });
-                collide<T>(fl, xq + 2, a, smag, [&](auto I, T v) { pend[I] = C::first(v, wgt<T>(I), scale); });
-                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rh[I] >> 16), wgt<T>(I), inv_scale); });
-                collide<T>(fl, xq + 3, a, smag, [&](auto I, T v) { oh[I] = C::second(pend[I], v, wgt<T>(I), scale); });
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rh[I] & 0xFFFFu), dsh(I), inv_scale); });
+                collide<NORM, T>(fl, xq + 2, a, smag, [&](auto I, T v) { pend[I] = C::first(v, esh(I), scale); });
+                static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rh[I] >> 16), dsh(I), inv_scale); });
+                collide<NORM, T>(fl, xq + 3, a, smag, [&](auto I, T v) { oh[I] = C::second(pend[I], v, esh(I), scale); });
             }
             if (xq < nx) {
                 static_for<0, Q>([&](auto I) {
@@ -468,9 +538,9 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             {
                 T fl[Q];
                 static_for<0, Q>([&](auto I) {
-                    fl[I] = C::decode((uint16_t)(raw[I] & 0xFFFFu), wgt<T>(I), inv_scale);
+                    fl[I] = C::decode((uint16_t)(raw[I] & 0xFFFFu), dsh(I), inv_scale);
                 });
-                collide<T>(fl, xa, a, smag, [&](auto I, T v) { pnd[I] = C::first(v, wgt<T>(I), scale); });
+                collide<NORM, T>(fl, xa, a, smag, [&](auto I, T v) { pnd[I] = C::first(v, esh(I), scale); });
             }
             if (x0 == 0) {                                               // inlet cell x = 0 (:67-71)
                 T u0 = (T)a.uin[z];
@@ -478,7 +548,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                 const T usq15 = T(1.5) * (u0 * u0);
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    const typename C::Pending e = C::first(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
+                    const typename C::Pending e = C::first(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale);
                     pnd[i] = (lane == 0) ? e : pnd[i];
                 });
                 inlet_done = true;
@@ -486,9 +556,9 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             {
                 T fl[Q];
                 static_for<0, Q>([&](auto I) {
-                    fl[I] = C::decode((uint16_t)(raw[I] >> 16), wgt<T>(I), inv_scale);
+                    fl[I] = C::decode((uint16_t)(raw[I] >> 16), dsh(I), inv_scale);
                 });
-                collide<T>(fl, xa + 1, a, smag, [&](auto I, T v) { out[I] = C::second(pnd[I], v, wgt<T>(I), scale); });
+                collide<NORM, T>(fl, xa + 1, a, smag, [&](auto I, T v) { out[I] = C::second(pnd[I], v, esh(I), scale); });
             }
             if (xa < nx) {
                 static_for<0, Q>([&](auto I) {
@@ -560,25 +630,25 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                     vb = wave_shl1(lo, mem.ld32(row, redge));
                     if (xl + 2 >= nx) vb = hi;                           // outlet: cell nx-1 pulls itself
                 }
-                fa[i] = C::decode(va, wgt<T>(i), inv_scale);
-                fb[i] = C::decode(vb, wgt<T>(i), inv_scale);
+                fa[i] = C::decode(va, dsh(I), inv_scale);
+                fb[i] = C::decode(vb, dsh(I), inv_scale);
             });
             uint32_t oa[Q];
-            collide<T>(fa, xa, a, smag, [&](auto I, T v) { oa[I] = C::encode(v, wgt<T>(I), scale); });
+            collide<NORM, T>(fa, xa, a, smag, [&](auto I, T v) { oa[I] = C::encode(v, esh(I), scale); });
             if (x0 == 0) {                                               // inlet cell x = 0 (:67-71)
                 T u0 = (T)a.uin[z];
                 if (a.gust) u0 = u0 * (T)a.gust[t];
                 const T usq15 = T(1.5) * (u0 * u0);
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    const uint32_t e = C::encode(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale);
+                    const uint32_t e = C::encode(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale);
                     oa[i] = (lane == 0) ? e : oa[i];
                 });
                 inlet_done = true;
             }
             uint64_t ob[Q];
-            collide<T>(fb, xa + 1, a, smag, [&](auto I, T v) {
-                ob[I] = (uint64_t)oa[I] | ((uint64_t)C::encode(v, wgt<T>(I), scale) << 32);
+            collide<NORM, T>(fb, xa + 1, a, smag, [&](auto I, T v) {
+                ob[I] = (uint64_t)oa[I] | ((uint64_t)C::encode(v, esh(I), scale) << 32);
             });
             if (xa < nx) {
                 static_for<0, Q>([&](auto I) {
@@ -627,7 +697,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             mem.template st<false>((size_t)i * NB + selfb, 0,
-                                   C::encode(feq<i, T>(T(1), u0, T(0), T(0), usq15), wgt<T>(i), scale));
+                                   C::encode(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale));
         });
     }
 }

@@ -11,6 +11,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libstonewall_lbm.so")
 
 SW_F32, SW_F16, SW_BF16, SW_F64 = 0, 1, 2, 3
 SW_Y_PERIODIC, SW_Y_FREESLIP = 0, 1
+SW_SHIFT_WEIGHTS, SW_SHIFT_INLET = 0, 1
+SHIFTS = {"weights": SW_SHIFT_WEIGHTS, "inlet": SW_SHIFT_INLET}
 DTYPES = {"f32": SW_F32, "f16": SW_F16, "bf16": SW_BF16, "f64": SW_F64}
 DTYPE_BYTES = {SW_F32: 4, SW_F16: 2, SW_BF16: 2, SW_F64: 8}
 
@@ -25,14 +27,15 @@ SIGNATURES = {
     "sw_links_words": (_c_ll, [_c_int, _c_int, _c_int]),
     "sw_partials_per_step": (_c_ll, [_c_int, _c_int]),
     "sw_prepare_links": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _p]),
-    "sw_init": (_c_int, [_p, _c_int, _p, _p, _c_int, _c_int, _c_int, _c_flt, _c_uint, _c_flt, _p]),
+    "sw_init": (_c_int, [_p, _c_int, _p, _p, _c_int, _c_int, _c_int, _c_flt, _c_uint, _c_flt, _p, _p]),
     "sw_step": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
-                         _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _c_int, _p]),
+                         _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p]),
     "sw_step_variants": (_c_int, []),
     "sw_set_step_variant": (_c_int, [_c_int, _c_int]),
     "sw_get_step_variant": (_c_int, [_c_int]),
     "sw_force_reduce": (_c_int, [_p, _c_int, _c_int, _p, _c_int, _c_int, _c_int, _p, _p]),
-    "sw_macro": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_flt, _p, _p, _p]),
+    "sw_macro": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_flt, _p, _p, _p, _p]),
+    "sw_shift_table": (_c_int, [_p, _c_int, _c_int, _p, _p]),
     "sw_qcoarse": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "sw_selftest_division": (_c_int, [_p, _p]),
     "sw_batch_table_bytes": (_c_ll, [_c_int]),
@@ -53,7 +56,7 @@ SIGNATURES = {
 class SwInstance(ctypes.Structure):
     """Mirror of struct sw_instance (include/stonewall_lbm.h)."""
     _fields_ = [("f0", _p), ("f1", _p), ("links", _p), ("uin", _p), ("gust", _p), ("partials", _p),
-                ("force", _p), ("nu0", _c_dbl), ("cs", _c_dbl), ("nu_sponge", _c_dbl), ("scale", _c_flt),
+                ("force", _p), ("shift", _p), ("nu0", _c_dbl), ("cs", _c_dbl), ("nu_sponge", _c_dbl), ("scale", _c_flt),
                 ("nx", _c_int), ("ny", _c_int), ("nz", _c_int), ("ring", _c_int), ("x_sponge", _c_int),
                 ("ybc", _c_int), ("dtype", _c_int), ("t", _c_int), ("t_end", _c_int), ("parity", _c_int),
                 ("item0", _c_int), ("reserved", _c_int)]

@@ -18,14 +18,14 @@ import torch
 
 from . import _native as nat
 from .lattice import Q, flag_from_mask
-from .lbm3d import DEFAULT_SCALE, TORCH_STORAGE, _stream
+from .lbm3d import DEFAULT_SCALE, DEFAULT_SHIFT, TORCH_STORAGE, _stream, storage_shift
 
 
 class Lattice:
     """Device resources of one lattice of a batch (the per-instance part of Simulation)."""
 
     def __init__(self, flag, uin, *, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, ybc="freeslip", dtype="f32",
-                 scale=None, gust=None, n_steps=1000, ring=32, device="cuda", tag=None):
+                 scale=None, gust=None, n_steps=1000, ring=32, device="cuda", tag=None, shift=None):
         lib = nat.lib()
         flag = np.asarray(flag)
         if flag.dtype == bool:
@@ -49,6 +49,8 @@ class Lattice:
         st = TORCH_STORAGE[self.dtype]
         self.f = [torch.empty(Q * self.N, dtype=st, device=dev), torch.zeros(Q * self.N, dtype=st, device=dev)]
         self.uin = torch.from_numpy(np.ascontiguousarray(uin, np.float32)).to(dev)
+        self.shift_kind = DEFAULT_SHIFT if shift is None else shift
+        self.shift = storage_shift(self.uin, self.dtype, self.shift_kind)
         self.gust = None
         if gust is not None:
             self.gust = torch.from_numpy(np.ascontiguousarray(gust[:self.n_steps], np.float32)).to(dev)
@@ -62,11 +64,13 @@ class Lattice:
     def nbytes(self):
         return sum(x.numel() * x.element_size() for x in (self.links, self.f[0], self.f[1], self.uin, self.partials,
                                                           self.force)) + \
-            (self.gust.numel() * 4 if self.gust is not None else 0)
+            (self.gust.numel() * 4 if self.gust is not None else 0) + \
+            (self.shift.numel() * 4 if self.shift is not None else 0)
 
     def init(self, noise=0.0, seed=0):
         nat.call("sw_init", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.uin.data_ptr(), self.nx,
-                 self.ny, self.nz, float(noise), int(seed) & 0xFFFFFFFF, self.scale, _stream())
+                 self.ny, self.nz, float(noise), int(seed) & 0xFFFFFFFF, self.scale,
+                 self.shift.data_ptr() if self.shift is not None else None, _stream())
         self.t, self.parity = 0, 0
         return self
 
@@ -76,6 +80,7 @@ class Lattice:
         e.links, e.uin = self.links.data_ptr(), self.uin.data_ptr()
         e.gust = self.gust.data_ptr() if self.gust is not None else None
         e.partials, e.force = self.partials.data_ptr(), self.force.data_ptr()
+        e.shift = self.shift.data_ptr() if self.shift is not None else None
         e.nu0, e.cs, e.nu_sponge, e.scale = self.nu0, self.cs, self.nu_sponge, self.scale
         e.nx, e.ny, e.nz = self.nx, self.ny, self.nz
         e.ring, e.x_sponge, e.ybc, e.dtype = self.ring, self.x_sponge, self.ybc, self.dtype

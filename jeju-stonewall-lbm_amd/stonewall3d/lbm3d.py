@@ -27,6 +27,41 @@ TORCH_STORAGE = {nat.SW_F32: torch.float32, nat.SW_F16: torch.float16, nat.SW_BF
                  nat.SW_F64: torch.float64}
 DEFAULT_SCALE = {nat.SW_F32: 1.0, nat.SW_F64: 1.0, nat.SW_F16: 1024.0, nat.SW_BF16: 1.0}
 BYTES_PER_UPDATE = {nat.SW_F32: 152, nat.SW_F16: 76, nat.SW_BF16: 76, nat.SW_F64: 304}
+# offsets of the 16-bit forms, which store (f_i / w_i - sigma_c(z)) * scale: sigma is the
+# normalised equilibrium of the inlet velocity at each height ("inlet"), or 1 ("weights",
+# i.e. (f_i - w_i) / w_i)
+DEFAULT_SHIFT = "inlet"
+W32 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, np.float32)
+XCLASS = np.array([0, 1, 2, 0, 0, 0, 0, 1, 2, 1, 2, 1, 2, 1, 2, 0, 0, 0, 0])   # c_x = 0, +1, -1
+
+
+def storage_shift(uin_dev, dtype, kind=None):
+    """Device sigma table (nz, 4) float32 of a 16-bit storage form (sigma_c(z) for c_x = 0,
+    +1, -1 and a zero pad), or None for FP32 / FP64 (the kernels then ignore it)."""
+    kind = DEFAULT_SHIFT if kind is None else kind
+    if kind not in nat.SHIFTS:
+        raise ValueError(f"shift must be one of {sorted(nat.SHIFTS)}")
+    if dtype not in (nat.SW_F16, nat.SW_BF16):
+        return None
+    nz = uin_dev.numel()
+    t = torch.empty((nz, 4), dtype=torch.float32, device=uin_dev.device)
+    nat.call("sw_shift_table", uin_dev.data_ptr(), nz, nat.SHIFTS[kind], t.data_ptr(), _stream())
+    return t
+
+
+def decoded(f_flat, dtype, scale, shift, nz):
+    """float32 (19, N) population values f_i of a device state f[i][z][y][x]; the 16-bit
+    forms decode phi_i = sigma + d / scale and return w_i * phi_i."""
+    f = f_flat.view(Q, -1)
+    if dtype not in (nat.SW_F16, nat.SW_BF16):
+        return f
+    dev = f.device
+    if shift is None:
+        sig = torch.ones((Q, nz, 1), dtype=torch.float32, device=dev)
+    else:
+        sig = shift[:, torch.from_numpy(XCLASS).to(dev)].t().contiguous()[:, :, None]
+    phi = sig + f.float().view(Q, nz, -1) * np.float32(1.0 / scale)
+    return (torch.from_numpy(W32).to(dev)[:, None, None] * phi).view(Q, -1)
 
 
 def _ptr(t):
@@ -48,13 +83,15 @@ class Simulation:
                 None = nx - 3H is the caller's choice, so here None means no sponge.
     ybc       : "periodic" (the reference) or "freeslip" side walls.
     dtype     : "f32" | "f16" | "bf16" | "f64" population storage.
+    shift     : 16-bit forms: "inlet" (default; store f_i / w_i - feq_i(1, (uin[z], 0, 0)) / w_i)
+                or "weights" (store f_i / w_i - 1), times the scale.
     gust      : optional (n_steps,) float32 inlet factor per step.
     n_steps   : steps this run may take (sizes the force record and the gust table).
     ring      : steps per force-partials ring (reduced every `ring` steps).
     """
 
     def __init__(self, flag, uin, *, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, ybc="periodic", dtype="f32",
-                 scale=None, gust=None, n_steps=1000, ring=64, device="cuda", record_force=True):
+                 scale=None, gust=None, n_steps=1000, ring=64, device="cuda", record_force=True, shift=None):
         nat.lib()
         flag = np.asarray(flag)
         if flag.dtype == bool:
@@ -86,6 +123,8 @@ class Simulation:
         st = TORCH_STORAGE[self.dtype]
         self.f = [torch.empty(Q * self.N, dtype=st, device=dev), torch.zeros(Q * self.N, dtype=st, device=dev)]
         self.uin = torch.from_numpy(uin).to(dev)
+        self.shift_kind = DEFAULT_SHIFT if shift is None else shift
+        self.shift = storage_shift(self.uin, self.dtype, self.shift_kind)
         self.gust = None
         if gust is not None:
             g = np.ascontiguousarray(gust, np.float32)
@@ -110,7 +149,7 @@ class Simulation:
     def init(self, noise=0.0, seed=0):
         """k_init (lbm3d.py:39-54, launched at :340)."""
         nat.call("sw_init", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.uin.data_ptr(), self.nx,
-                 self.ny, self.nz, float(noise), int(seed) & 0xFFFFFFFF, self.scale, _stream())
+                 self.ny, self.nz, float(noise), int(seed) & 0xFFFFFFFF, self.scale, _ptr(self.shift), _stream())
         self.t = 0
         self._pending = 0
         return self
@@ -125,7 +164,7 @@ class Simulation:
             rc = lib.sw_step(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
                              self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
                              self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
-                             _ptr(self.partials), self.ring, s)
+                             _ptr(self.shift), _ptr(self.partials), self.ring, s)
             nat.check(rc, "sw_step")
             self.f[0], self.f[1] = self.f[1], self.f[0]
             self.t += 1
@@ -167,7 +206,8 @@ class Simulation:
         return out
 
     def populations(self):
-        """Raw stored populations in the reference layout (19, nx, ny, nz) (storage dtype)."""
+        """Raw stored populations in the reference layout (19, nx, ny, nz) (storage dtype;
+        the 16-bit forms hold (f_i / w_i - sigma) * scale, see shift_table())."""
         eb = nat.DTYPE_BYTES[self.dtype]
         out = self._to_ref(self.f[0], Q, eb).cpu()
         if self.dtype == nat.SW_BF16:
@@ -175,6 +215,10 @@ class Simulation:
         if self.dtype == nat.SW_F16:
             return out.numpy().view(np.uint16).reshape(Q, self.nx, self.ny, self.nz)
         return out.numpy().reshape(Q, self.nx, self.ny, self.nz)
+
+    def shift_table(self):
+        """(nz, 4) float32 sigma table of a 16-bit form (None for FP32 / FP64)."""
+        return None if self.shift is None else self.shift.cpu().numpy()
 
     def load_populations(self, arr):
         """Inverse of populations(): raw storage values in the reference layout."""
@@ -195,7 +239,7 @@ class Simulation:
         if self.U is None:
             self.U = torch.empty(4 * self.N, dtype=torch.float32, device=self.device)
         nat.call("sw_macro", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.nx, self.ny, self.nz,
-                 self.scale, self.U.data_ptr(), None, _stream())
+                 self.scale, _ptr(self.shift), self.U.data_ptr(), None, _stream())
         return self._to_ref(self.U, 4, 4).cpu().numpy().reshape(4, self.nx, self.ny, self.nz)
 
     def accumulate(self):
@@ -203,7 +247,7 @@ class Simulation:
         if self.Uacc is None:
             self.Uacc = torch.zeros(4 * self.N, dtype=torch.float32, device=self.device)
         nat.call("sw_macro", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.nx, self.ny, self.nz,
-                 self.scale, None, self.Uacc.data_ptr(), _stream())
+                 self.scale, _ptr(self.shift), None, self.Uacc.data_ptr(), _stream())
         self.n_acc += 1
 
     def accumulator(self):
@@ -243,12 +287,7 @@ class Simulation:
 
     def checksum(self):
         """FP64 fluid-only sums (sum f, sum (i+1) f) of the decoded populations, on the device."""
-        f = self.f[0].view(Q, self.N)
-        w = torch.tensor([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=torch.float32, device=self.device)
-        if self.dtype in (nat.SW_F16, nat.SW_BF16):
-            v = w[:, None] + f.float() * np.float32(1.0 / self.scale)
-        else:
-            v = f
+        v = decoded(self.f[0], self.dtype, self.scale, self.shift, self.nz)
         fluid = (self.links[:self.N] & (1 << 31)) == 0
         v = torch.where(fluid[None, :], v.double(), torch.zeros((), dtype=torch.float64, device=self.device))
         idx = torch.arange(1, Q + 1, dtype=torch.float64, device=self.device)[:, None]
@@ -262,7 +301,7 @@ class Simulation:
         if self.U is None:
             self.U = torch.empty(4 * self.N, dtype=torch.float32, device=self.device)
         nat.call("sw_macro", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.nx, self.ny, self.nz,
-                 self.scale, self.U.data_ptr(), None, _stream())
+                 self.scale, _ptr(self.shift), self.U.data_ptr(), None, _stream())
         U = self.U.view(4, self.nz, self.ny, self.nx)
         xz = U[1:, :, y_mid, :].transpose(1, 2).to(torch.float16).cpu().numpy()
         xy = U[1:, z_half, :, :].transpose(1, 2).to(torch.float16).cpu().numpy()
@@ -275,7 +314,7 @@ class Simulation:
         if self.U is None:
             self.U = torch.empty(4 * self.N, dtype=torch.float32, device=self.device)
         nat.call("sw_macro", self.f[0].data_ptr(), self.dtype, self.links.data_ptr(), self.nx, self.ny, self.nz,
-                 self.scale, self.U.data_ptr(), None, _stream())
+                 self.scale, _ptr(self.shift), self.U.data_ptr(), None, _stream())
         q = torch.empty(int(np.prod(dims)), dtype=torch.float32, device=self.device)
         nat.call("sw_qcoarse", self.U.data_ptr(), self.links.data_ptr(), self.nx, self.ny, self.nz, int(x0), int(ds),
                  int(dims[0]), int(dims[1]), int(dims[2]), q.data_ptr(), _stream())
@@ -320,6 +359,8 @@ def _parser():
     ap.add_argument("--no-resume", action="store_true")
     # additions
     ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"], help="population storage")
+    ap.add_argument("--shift", default=DEFAULT_SHIFT, choices=sorted(nat.SHIFTS),
+                    help="16-bit storage offset: inlet equilibrium per height, or the weights")
     ap.add_argument("--ybc", default="periodic", choices=["periodic", "freeslip"], help="side boundaries")
     ap.add_argument("--gust-seed", type=int, default=None, help="gusting inlet from this seed (default: steady)")
     ap.add_argument("--noise", type=float, default=0.02, help="init velocity noise amplitude (lbm3d.py:340)")
@@ -354,7 +395,7 @@ def main(argv=None):
     wall, phi = build_wall(a.porosity, H, nx, ny, nz, X0, T, seed=a.seed)
     gust = gust_table(make_gust(a.gust_seed), n_total) if a.gust_seed is not None else None
     sim = Simulation(wall, su.uin, nu0=su.nu0, cs=a.Cs, x_sponge=su.x_sponge, nu_sponge=0.05, ybc=a.ybc,
-                     dtype=a.dtype, gust=gust, n_steps=n_total, device=a.device)
+                     dtype=a.dtype, gust=gust, n_steps=n_total, device=a.device, shift=a.shift)
     y_mid, z_half = ny // 2, int(round(0.5 * H)) + 1
     ds3 = a.snap3d_ds if a.snap3d_ds > 0 else max(1, int(round(H / 13)))            # lbm3d.py:308-313
     q_x0 = max(1, X0 - int(round(0.5 * H)))
@@ -365,6 +406,8 @@ def main(argv=None):
     sig = dict(grid=[nx, ny, nz], H=H, porosity=a.porosity, seed=a.seed, Re=a.Re, U0=a.U0, Cs=a.Cs, n_spin=n_spin,
                n_total=n_total, snap_every=snap_every, snap3d=[a.snap3d_every, ds3], dtype=a.dtype,
                ybc=a.ybc, gust_seed=a.gust_seed)
+    if a.dtype in ("f16", "bf16"):
+        sig["shift"] = a.shift
     snaps_xz, snaps_xy, snaps_q, t_start = [], [], [], 0
     if os.path.isdir(ckpt) and not a.no_resume:
         t_start, n_acc, snaps = checkpoint.load(sim, ckpt, sig)
@@ -436,7 +479,7 @@ def main(argv=None):
                                       every=a.snap3d_every * snap_every),
                              nx=nx, ny=ny, nz=nz, UH=su.UH, U0=a.U0, Re=a.Re, nu=su.nu0, Cs=a.Cs, n_spin=n_spin,
                              n_avg=n_total - n_spin, steps_per_unit=su.t_unit, snap_every=snap_every, y_mid=y_mid,
-                             z_half=z_half, seed=a.seed, device=a.device, dtype=a.dtype, ybc=a.ybc,
+                             z_half=z_half, seed=a.seed, device=a.device, dtype=a.dtype, shift=sim.shift_kind, ybc=a.ybc,
                              gust_seed=a.gust_seed, wall_seconds=time.time() - t0)))
     s = slice(n_spin, None)
     if n_total > n_spin:

@@ -24,9 +24,10 @@ def lib():
             build()
         L = ctypes.CDLL(LIB)
         p, i, f, d, u = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_uint
-        L.oracle_init_f32.argtypes = [p, i, p, p, i, i, i, f, u, f]
-        L.oracle_step_f32.argtypes = [p, p, i, p, p, f, i, i, i, f, f, i, f, i, f, p]
-        L.oracle_macro_f32.argtypes = [p, i, p, i, i, i, f, p, p]
+        L.oracle_init_f32.argtypes = [p, i, p, p, i, i, i, f, u, f, p]
+        L.oracle_step_f32.argtypes = [p, p, i, p, p, f, i, i, i, f, f, i, f, i, f, p, p]
+        L.oracle_macro_f32.argtypes = [p, i, p, i, i, i, f, p, p, p]
+        L.oracle_shift_table.argtypes = [p, i, i, p]
         L.oracle_init_f64.argtypes = [p, p, p, i, i, i, f, u]
         L.oracle_step_f64.argtypes = [p, p, p, p, d, i, i, i, d, d, i, d, i, p]
         L.oracle_macro_f64.argtypes = [p, p, i, i, i, p]
@@ -65,11 +66,31 @@ def tables():
     return dict(cx=cx, cy=cy, cz=cz, opp=opp, mirz=mirz, miry=miry, w=w)
 
 
+SHIFT_KINDS = {"weights": 0, "inlet": 1}
+
+
+def shift_table(uin, kind="inlet"):
+    """(nz, 4) float32 offset table of the 16-bit forms: sigma_c(z) for c_x = 0, +1, -1 (and
+    a zero pad), the equilibrium feq_i(1, (U, 0, 0)) / w_i of U = uin[z] ("inlet") or U = 0
+    ("weights", sigma = 1)."""
+    uin = np.ascontiguousarray(uin, np.float32)
+    out = np.zeros((uin.shape[0], 4), np.float32)
+    lib().oracle_shift_table(_ptr(uin), uin.shape[0], SHIFT_KINDS[kind], _ptr(out))
+    return out
+
+
+def _shift_ptr(shift):
+    if shift is None:
+        return None
+    assert shift.dtype == np.float32 and shift.flags.c_contiguous and shift.shape[1] == 4
+    return _ptr(shift)
+
+
 def storage_dtype(st):
     return np.float32 if st == "f32" else np.uint16
 
 
-def init(flag, uin, storage="f32", noise=0.0, seed=0, scale=1.0):
+def init(flag, uin, storage="f32", noise=0.0, seed=0, scale=1.0, shift=None):
     flag = np.ascontiguousarray(flag, np.uint8)
     nx, ny, nz = flag.shape
     uin = np.ascontiguousarray(uin, np.float32)
@@ -78,14 +99,16 @@ def init(flag, uin, storage="f32", noise=0.0, seed=0, scale=1.0):
         lib().oracle_init_f64(_ptr(f), _ptr(flag), _ptr(uin), nx, ny, nz, float(np.float32(noise)), seed)
         return f
     f = np.zeros((19, nx, ny, nz), storage_dtype(storage))
-    lib().oracle_init_f32(_ptr(f), ST[storage], _ptr(flag), _ptr(uin), nx, ny, nz, noise, seed, scale)
+    lib().oracle_init_f32(_ptr(f), ST[storage], _ptr(flag), _ptr(uin), nx, ny, nz, noise, seed, scale,
+                          _shift_ptr(shift))
     return f
 
 
 def step(f_in, flag, uin, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, gust=1.0, ybc=0, storage="f32",
-         scale=1.0, f_out=None):
+         scale=1.0, f_out=None, shift=None):
     """One k_step; returns (f_out, force_z (nz, 2) float64).  f_out is allocated if None;
-    its solid cells keep whatever they held (the reference never writes them)."""
+    its solid cells keep whatever they held (the reference never writes them).  shift: the
+    (nz, 19) offset table of the 16-bit forms (None = the weights)."""
     flag = np.ascontiguousarray(flag, np.uint8)
     nx, ny, nz = flag.shape
     if x_sponge is None:
@@ -99,12 +122,12 @@ def step(f_in, flag, uin, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, gust=1.0,
                                    nz, float(nu0), float(cs), int(x_sponge), float(nu_sponge), ybc, _ptr(force))
     else:
         rc = lib().oracle_step_f32(_ptr(f_in), _ptr(f_out), ST[storage], _ptr(flag), _ptr(uin), gust, nx, ny, nz,
-                                   nu0, cs, int(x_sponge), nu_sponge, ybc, scale, _ptr(force))
+                                   nu0, cs, int(x_sponge), nu_sponge, ybc, scale, _shift_ptr(shift), _ptr(force))
     assert rc == 0
     return f_out, force
 
 
-def macro(f, flag, storage="f32", scale=1.0, acc=None):
+def macro(f, flag, storage="f32", scale=1.0, acc=None, shift=None):
     flag = np.ascontiguousarray(flag, np.uint8)
     nx, ny, nz = flag.shape
     if storage == "f64":
@@ -112,22 +135,34 @@ def macro(f, flag, storage="f32", scale=1.0, acc=None):
         lib().oracle_macro_f64(_ptr(f), _ptr(flag), nx, ny, nz, _ptr(U))
         return U
     U = np.zeros((4, nx, ny, nz), np.float32)
-    lib().oracle_macro_f32(_ptr(f), ST[storage], _ptr(flag), nx, ny, nz, scale, _ptr(U),
+    lib().oracle_macro_f32(_ptr(f), ST[storage], _ptr(flag), nx, ny, nz, scale, _shift_ptr(shift), _ptr(U),
                            _ptr(acc) if acc is not None else None)
     return U
 
 
-def decode(f, storage, scale=1.0):
-    """Populations of a storage form as float32 values w_i + d / scale."""
+def sigma_per_direction(shift, nz):
+    """(19, nz) float32 sigma of each direction's c_x class at each height."""
+    from . import np_lbm
+    cls = np.where(np_lbm.C[:, 0] == 0, 0, np.where(np_lbm.C[:, 0] > 0, 1, 2))
+    if shift is None:
+        return np.ones((19, nz), np.float32)
+    return np.ascontiguousarray(shift[:, cls].T)
+
+
+def decode(f, storage, scale=1.0, shift=None):
+    """Populations of a storage form as float32 values.  16-bit forms: f_i = w_i * phi_i with
+    phi_i = sigma_c(z) + d / scale (sigma = 1 when shift is None)."""
     from . import np_lbm
     if storage in ("f32", "f64"):
         return f
-    w = np_lbm.W32.reshape(19, 1, 1, 1)
+    nz = f.shape[3]
+    sig = sigma_per_direction(shift, nz)[:, None, None, :]
     if storage == "f16":
         d = f.view(np.float16).astype(np.float32)
     else:
         d = (f.astype(np.uint32) << 16).view(np.float32)
-    return w + d * np.float32(1.0 / scale)
+    phi = sig + d * np.float32(1.0 / scale)
+    return np_lbm.W32.reshape(19, 1, 1, 1) * phi
 
 
 def qcoarse(U, flag, x0, ds, dims):

@@ -24,8 +24,10 @@
  *   - gust:     the inlet speed is uin[z]*gust (gust = 1 reproduces lbm3d.py:68);
  *   - ybc = 1:  free-slip side walls (y-mirror, by analogy with the z-mirror
  *               at lbm3d.py:84-86); ybc = 0 is the reference's periodic y (:79);
- *   - storage:  populations stored as FP16 / BF16 shifted by their weight,
- *               value = w_i + decode(h) / scale, with all arithmetic in FP32;
+ *   - storage:  FP16 / BF16 populations (see "16-bit storage forms" below): the
+ *               weight-normalised population phi_i = f_i / w_i is stored shifted by the
+ *               normalised equilibrium sigma_c(z) of a streamwise velocity U(z) (the inlet
+ *               profile, or U = 0), with the collision in FP32 on phi;
  *   - FP64:     the same algorithm with double everywhere;
  *   - force:    each cell's FP32 (fxw, fzw) exactly as lbm3d.py:89-101, summed per
  *               z level in FP64 in cell order instead of by FP32 atomics.
@@ -128,20 +130,7 @@ static float u01(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
 /* ---------------------------------------------------------------- storage codecs */
 enum { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2, ST_F64 = 3 };
 
-/* ================================================================ FP32 / shifted storage
- * One body for the three FP32-arithmetic storage forms. */
-static inline float load32(const void* f, size_t idx, int st, float w, float inv_scale) {
-    if (st == ST_F32) return ((const float*)f)[idx];
-    float d = (st == ST_F16) ? oracle_f16_to_f32(((const uint16_t*)f)[idx])
-                             : oracle_bf16_to_f32(((const uint16_t*)f)[idx]);
-    return w + d * inv_scale;
-}
-static inline void store32(void* f, size_t idx, int st, float v, float w, float scale) {
-    if (st == ST_F32) { ((float*)f)[idx] = v; return; }
-    float d = (v - w) * scale;
-    ((uint16_t*)f)[idx] = (st == ST_F16) ? oracle_f32_to_f16(d) : oracle_f32_to_bf16(d);
-}
-
+/* ================================================================ FP32 storage */
 /* feq_i, lbm3d.py:33-36, float32 */
 static inline float feq32(float w, float cx, float cy, float cz, float rho, float ux, float uy, float uz) {
     float cu = 3.0f * (cx * ux + cy * uy + cz * uz);
@@ -156,25 +145,38 @@ static inline double feq64(double w, double cx, double cy, double cz, double rho
 #define FIDX(x, y, z) ((((size_t)(x)) * ny + (size_t)(y)) * nz + (size_t)(z))
 
 /* k_init (lbm3d.py:39-54) with the PCG noise documented above. */
+static int init_h16(void* f, int st, const uint8_t* flag, const float* uin, int nx, int ny, int nz, float noise,
+                    uint32_t seed, float scale, const float* sig);
+static int step_h16(const void* f_in, void* f_out, int st, const uint8_t* flag, const float* uin, float gust,
+                    int nx, int ny, int nz, float nu0, float cs, int x_sponge, float nu_sponge, int ybc,
+                    float scale, const float* sig, double* force_z);
+static int macro_h16(const void* f, int st, const uint8_t* flag, int nx, int ny, int nz, float scale,
+                     const float* sig, float* U, float* acc);
+
+/* The init noise of lbm3d.py:46-52 (PCG stream documented above) for fluid cell (x, y, z). */
+static inline void init_velocity(uint32_t seed, int x, int y, int z, float u0, float noise,
+                                 float* ux, float* uy, float* uz) {
+    uint32_t key = ((uint32_t)x * 7919u + (uint32_t)y) * 7907u + (uint32_t)z;
+    uint32_t h0 = pcg_hash(seed ^ pcg_hash(key));
+    uint32_t h1 = pcg_hash(h0), h2 = pcg_hash(h1);
+    *ux = u0 * (1.0f + noise * (2.0f * u01(h0) - 1.0f));
+    *uy = u0 * noise * (2.0f * u01(h1) - 1.0f);
+    *uz = u0 * noise * (2.0f * u01(h2) - 1.0f);
+}
+
+/* shift: the sigma table of the 16-bit forms ([nz][4], NULL = U = 0); ignored for FP32. */
 int oracle_init_f32(void* f, int st, const uint8_t* flag, const float* uin, int nx, int ny, int nz,
-                    float noise, uint32_t seed, float scale) {
+                    float noise, uint32_t seed, float scale, const float* shift) {
+    if (st != ST_F32) return init_h16(f, st, flag, uin, nx, ny, nz, noise, seed, scale, shift);
     #pragma omp parallel for schedule(static)
     for (int x = 0; x < nx; x++)
         for (int y = 0; y < ny; y++)
             for (int z = 0; z < nz; z++) {
                 float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-                if (flag[FIDX(x, y, z)] == 0) {
-                    uint32_t key = ((uint32_t)x * 7919u + (uint32_t)y) * 7907u + (uint32_t)z;
-                    uint32_t h0 = pcg_hash(seed ^ pcg_hash(key));
-                    uint32_t h1 = pcg_hash(h0), h2 = pcg_hash(h1);
-                    float u0 = uin[z];
-                    ux = u0 * (1.0f + noise * (2.0f * u01(h0) - 1.0f));
-                    uy = u0 * noise * (2.0f * u01(h1) - 1.0f);
-                    uz = u0 * noise * (2.0f * u01(h2) - 1.0f);
-                }
+                if (flag[FIDX(x, y, z)] == 0) init_velocity(seed, x, y, z, uin[z], noise, &ux, &uy, &uz);
                 for (int i = 0; i < Q; i++)
-                    store32(f, IDX(i, x, y, z), st,
-                            feq32(W32(i), (float)CX[i], (float)CY[i], (float)CZ[i], 1.0f, ux, uy, uz), W32(i), scale);
+                    ((float*)f)[IDX(i, x, y, z)] = feq32(W32(i), (float)CX[i], (float)CY[i], (float)CZ[i], 1.0f,
+                                                         ux, uy, uz);
             }
     return 0;
 }
@@ -200,8 +202,10 @@ static inline void source(int i, int x, int y, int z, int nx, int ny, int nz, in
  * (fxw, fzw), summed in FP64 in x, y order per z (deterministic for any thread count). */
 int oracle_step_f32(const void* f_in, void* f_out, int st, const uint8_t* flag, const float* uin, float gust,
                     int nx, int ny, int nz, float nu0, float cs, int x_sponge, float nu_sponge, int ybc,
-                    float scale, double* force_z) {
-    const float inv_scale = 1.0f / scale;
+                    float scale, const float* shift, double* force_z) {
+    if (st != ST_F32)
+        return step_h16(f_in, f_out, st, flag, uin, gust, nx, ny, nz, nu0, cs, x_sponge, nu_sponge, ybc, scale,
+                        shift, force_z);
     double* part = NULL;
     if (force_z) {
         part = (double*)calloc((size_t)nx * nz * 2, sizeof(double));
@@ -216,9 +220,8 @@ int oracle_step_f32(const void* f_in, void* f_out, int st, const uint8_t* flag, 
                 if (x == 0) {                                                 /* :67-71 */
                     float u0 = uin[z] * gust;                                 /* gust = 1 is exact */
                     for (int i = 0; i < Q; i++)
-                        store32(f_out, IDX(i, x, y, z), st,
-                                feq32(W32(i), (float)CX[i], (float)CY[i], (float)CZ[i], 1.0f, u0, 0.0f, 0.0f),
-                                W32(i), scale);
+                        ((float*)f_out)[IDX(i, x, y, z)] =
+                            feq32(W32(i), (float)CX[i], (float)CY[i], (float)CZ[i], 1.0f, u0, 0.0f, 0.0f);
                     continue;
                 }
                 float fl[Q];
@@ -230,7 +233,7 @@ int oracle_step_f32(const void* f_in, void* f_out, int st, const uint8_t* flag, 
                     float val = 0.0f;
                     uint8_t fs = flag[FIDX(xs, ys, zs)];
                     if (fs != 0) {
-                        val = load32(f_in, IDX(OPP[i], x, y, z), st, W32(OPP[i]), inv_scale);
+                        val = ((const float*)f_in)[IDX(OPP[i], x, y, z)];
                         if (fs == 1) {
                             float fr = val - W32(i);
                             fxw = fxw - 2.0f * fr * (float)CX[i];
@@ -238,7 +241,7 @@ int oracle_step_f32(const void* f_in, void* f_out, int st, const uint8_t* flag, 
                             hit = 1;
                         }
                     } else {
-                        val = load32(f_in, IDX(j, xs, ys, zs), st, W32(j), inv_scale);
+                        val = ((const float*)f_in)[IDX(j, xs, ys, zs)];
                     }
                     fl[i] = val;
                 }
@@ -284,8 +287,7 @@ int oracle_step_f32(const void* f_in, void* f_out, int st, const uint8_t* flag, 
                     float qq = ((ex * ex - 1.0f / 3.0f) * nxx + (ey * ey - 1.0f / 3.0f) * nyy
                                 + (ez * ez - 1.0f / 3.0f) * nzz
                                 + 2.0f * (ex * ey * nxy + ex * ez * nxz + ey * ez * nyz));
-                    store32(f_out, IDX(i, x, y, z), st,
-                            feq32(W32(i), ex, ey, ez, rho, ux, uy, uz) + keep * W32(i) * 4.5f * qq, W32(i), scale);
+                    ((float*)f_out)[IDX(i, x, y, z)] = feq32(W32(i), ex, ey, ez, rho, ux, uy, uz) + keep * W32(i) * 4.5f * qq;
                 }
             }
     if (force_z) {
@@ -302,8 +304,8 @@ int oracle_step_f32(const void* f_in, void* f_out, int st, const uint8_t* flag, 
 
 /* k_macro + k_accum (lbm3d.py:158-188), FP32.  U and acc are (4, nx, ny, nz). */
 int oracle_macro_f32(const void* f, int st, const uint8_t* flag, int nx, int ny, int nz, float scale,
-                     float* U, float* acc) {
-    const float inv_scale = 1.0f / scale;
+                     const float* shift, float* U, float* acc) {
+    if (st != ST_F32) return macro_h16(f, st, flag, nx, ny, nz, scale, shift, U, acc);
     const size_t N = (size_t)nx * ny * nz;
     #pragma omp parallel for schedule(static)
     for (int x = 0; x < nx; x++)
@@ -316,7 +318,7 @@ int oracle_macro_f32(const void* f, int st, const uint8_t* flag, int nx, int ny,
                 } else {
                     float rho = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
                     for (int i = 0; i < Q; i++) {
-                        float v = load32(f, IDX(i, x, y, z), st, W32(i), inv_scale);
+                        float v = ((const float*)f)[IDX(i, x, y, z)];
                         rho += v;
                         mx += v * (float)CX[i];
                         my += v * (float)CY[i];
@@ -521,6 +523,216 @@ int oracle_qcoarse(const float* U, const uint8_t* flag, int nx, int ny, int nz, 
                             }
                         }
                 out[((size_t)i * qy + j) * qz + k] = best;
+            }
+    return 0;
+}
+
+/* ================================================================ 16-bit storage forms
+ * No counterpart in the reference (it stores FP32).  These forms store the weight-normalised
+ * population phi_i = f_i / w_i shifted by sigma_c(z) = feq_i(1, (U(z), 0, 0)) / w_i, the
+ * normalised equilibrium of a streamwise velocity U(z), which depends on the direction only
+ * through c = c_x,i:
+ *     sigma_c = ((1 + 3cU) + (0.5 * 3cU) * 3cU) - 1.5 U^2          (lbm3d.py:35-36 order)
+ * as q_i = RN16((phi_i - sigma_c(z)) * scale), scale a power of two, sigma from a table
+ * [nz][4] = (sigma_0, sigma_+1, sigma_-1, 0) (NULL: U = 0, sigma = 1, i.e. (f_i - w_i) / w_i).
+ * With U(z) the inlet profile the stored values are small wherever the flow is close to
+ * the inlet profile, and so is their rounding error.
+ *
+ * The step's FP32 arithmetic on phi, this build's own definition for these forms:
+ *   phi_i    = fmaf(d_i, 1/scale, sigma(z_src, c_x of the stored direction))  (one rounding)
+ *   rho      = fmaf chain over i = 0..18 of w_i * phi_i, starting from +0
+ *   m_a      = fmaf chain over i with c_ia != 0 of (c_ia w_i) * phi_i, from +0
+ *   Pi_ab    = fmaf chain over i with c_ia c_ib != 0 of (c_ia c_ib w_i) * phi_i, from +0
+ *   u, N, nu, tau, keep: exactly as the FP32 step (lbm3d.py:128-147)
+ *   phi_out  = fmaf(rho, E_i, (keep * 4.5) * qq_i),  E_i = ((1 + cu) + (0.5 cu) cu) - 1.5 u.u
+ *              (feq_i / w_i, lbm3d.py:35-36 order; qq_i as lbm3d.py:150-154)
+ *   inlet    = E_i at rho = 1, u = (uin[z] gust, 0, 0)
+ *   wall force per stone link: fr2 = (2 w_i) * (phi_bb - 1) = 2 (f - w_i) (lbm3d.py:92-94)
+ */
+static inline int sclass(int cx) { return cx == 0 ? 0 : (cx > 0 ? 1 : 2); }
+static inline float SIG(const float* sig, int z, int cx) { return sig ? sig[(size_t)z * 4 + sclass(cx)] : 1.0f; }
+
+static inline float dec16(const void* f, size_t idx, int st, float s, float inv_scale) {
+    uint16_t h = ((const uint16_t*)f)[idx];
+    float d = (st == ST_F16) ? oracle_f16_to_f32(h) : oracle_bf16_to_f32(h);
+    return fmaf(d, inv_scale, s);
+}
+static inline void enc16(void* f, size_t idx, int st, float v, float s, float scale) {
+    float d = (v - s) * scale;
+    ((uint16_t*)f)[idx] = (st == ST_F16) ? oracle_f32_to_f16(d) : oracle_f32_to_bf16(d);
+}
+/* feq_i / w_i of lbm3d.py:35-36 (rho = 1), usq15 = 1.5 * ((ux ux + uy uy) + uz uz) */
+static inline float E32(float cx, float cy, float cz, float ux, float uy, float uz, float usq15) {
+    float cu = 3.0f * (cx * ux + cy * uy + cz * uz);
+    return 1.0f + cu + 0.5f * cu * cu - usq15;
+}
+
+/* sigma table [nz][4] of the 16-bit forms: kind 1 = U(z) = uin[z] (the inlet equilibrium),
+ * kind 0 = U = 0 (sigma = 1). */
+int oracle_shift_table(const float* uin, int nz, int kind, float* out) {
+    for (int z = 0; z < nz; z++) {
+        float U = kind == 1 ? uin[z] : 0.0f;
+        float usq15 = 1.5f * (U * U + 0.0f * 0.0f + 0.0f * 0.0f);
+        out[(size_t)z * 4 + 0] = E32(0.0f, 1.0f, 0.0f, U, 0.0f, 0.0f, usq15);
+        out[(size_t)z * 4 + 1] = E32(1.0f, 0.0f, 0.0f, U, 0.0f, 0.0f, usq15);
+        out[(size_t)z * 4 + 2] = E32(-1.0f, 0.0f, 0.0f, U, 0.0f, 0.0f, usq15);
+        out[(size_t)z * 4 + 3] = 0.0f;
+    }
+    return 0;
+}
+
+static int init_h16(void* f, int st, const uint8_t* flag, const float* uin, int nx, int ny, int nz, float noise,
+                    uint32_t seed, float scale, const float* sig) {
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+                if (flag[FIDX(x, y, z)] == 0) init_velocity(seed, x, y, z, uin[z], noise, &ux, &uy, &uz);
+                float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+                for (int i = 0; i < Q; i++)
+                    enc16(f, IDX(i, x, y, z), st, E32((float)CX[i], (float)CY[i], (float)CZ[i], ux, uy, uz, usq15),
+                          SIG(sig, z, CX[i]), scale);
+            }
+    return 0;
+}
+
+/* rho and momentum of 19 normalised populations (the step's chains) */
+static inline void moments_h16(const float* ph, float* rho, float* mx, float* my, float* mz) {
+    float r = 0.0f, a = 0.0f, b = 0.0f, c = 0.0f;
+    for (int i = 0; i < Q; i++) {
+        float w = W32(i);
+        r = fmaf(w, ph[i], r);
+        if (CX[i]) a = fmaf((float)CX[i] * w, ph[i], a);
+        if (CY[i]) b = fmaf((float)CY[i] * w, ph[i], b);
+        if (CZ[i]) c = fmaf((float)CZ[i] * w, ph[i], c);
+    }
+    *rho = r; *mx = a; *my = b; *mz = c;
+}
+
+static int step_h16(const void* f_in, void* f_out, int st, const uint8_t* flag, const float* uin, float gust,
+                    int nx, int ny, int nz, float nu0, float cs, int x_sponge, float nu_sponge, int ybc,
+                    float scale, const float* sig, double* force_z) {
+    const float inv_scale = 1.0f / scale;
+    double* part = NULL;
+    if (force_z) {
+        part = (double*)calloc((size_t)nx * nz * 2, sizeof(double));
+        if (!part) return -1;
+    }
+    const float sq2 = sqrtf(2.0f);
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                if (flag[FIDX(x, y, z)] != 0) continue;
+                if (x == 0) {
+                    float u0 = uin[z] * gust;
+                    float usq15 = 1.5f * (u0 * u0 + 0.0f * 0.0f + 0.0f * 0.0f);
+                    for (int i = 0; i < Q; i++)
+                        enc16(f_out, IDX(i, x, y, z), st,
+                              E32((float)CX[i], (float)CY[i], (float)CZ[i], u0, 0.0f, 0.0f, usq15),
+                              SIG(sig, z, CX[i]), scale);
+                    continue;
+                }
+                float ph[Q];
+                float fxw = 0.0f, fzw = 0.0f;
+                int hit = 0;
+                for (int i = 0; i < Q; i++) {
+                    int xs, ys, zs, j;
+                    source(i, x, y, z, nx, ny, nz, ybc, &xs, &ys, &zs, &j);
+                    float v;
+                    uint8_t fs = flag[FIDX(xs, ys, zs)];
+                    if (fs != 0) {
+                        v = dec16(f_in, IDX(OPP[i], x, y, z), st, SIG(sig, z, CX[OPP[i]]), inv_scale);
+                        if (fs == 1) {
+                            float fr2 = (2.0f * W32(i)) * (v - 1.0f);
+                            fxw = fxw - fr2 * (float)CX[i];
+                            fzw = fzw - fr2 * (float)CZ[i];
+                            hit = 1;
+                        }
+                    } else {
+                        v = dec16(f_in, IDX(j, xs, ys, zs), st, SIG(sig, zs, CX[j]), inv_scale);
+                    }
+                    ph[i] = v;
+                }
+                if (hit && part) {
+                    part[((size_t)x * nz + z) * 2 + 0] += (double)fxw;
+                    part[((size_t)x * nz + z) * 2 + 1] += (double)fzw;
+                }
+                float rho, mx, my, mz;
+                moments_h16(ph, &rho, &mx, &my, &mz);
+                float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
+                for (int i = 0; i < Q; i++) {
+                    float w = W32(i);
+                    if (CX[i]) pxx = fmaf(w, ph[i], pxx);
+                    if (CY[i]) pyy = fmaf(w, ph[i], pyy);
+                    if (CZ[i]) pzz = fmaf(w, ph[i], pzz);
+                    if (CX[i] * CY[i] != 0) pxy = fmaf((float)(CX[i] * CY[i]) * w, ph[i], pxy);
+                    if (CX[i] * CZ[i] != 0) pxz = fmaf((float)(CX[i] * CZ[i]) * w, ph[i], pxz);
+                    if (CY[i] * CZ[i] != 0) pyz = fmaf((float)(CY[i] * CZ[i]) * w, ph[i], pyz);
+                }
+                float ux = mx / rho, uy = my / rho, uz = mz / rho;
+                float third = rho / 3.0f;
+                float nxx = pxx - third - rho * ux * ux;
+                float nyy = pyy - third - rho * uy * uy;
+                float nzz = pzz - third - rho * uz * uz;
+                float nxy = pxy - rho * ux * uy;
+                float nxz = pxz - rho * ux * uz;
+                float nyz = pyz - rho * uy * uz;
+                float nu = nu0;
+                if (x > x_sponge) {
+                    float s = (float)(x - x_sponge) / (float)(nx - x_sponge);
+                    nu = nu0 + nu_sponge * s * s;
+                }
+                float tau0 = 3.0f * nu + 0.5f;
+                float qn = sqrtf(nxx * nxx + nyy * nyy + nzz * nzz + 2.0f * (nxy * nxy + nxz * nxz + nyz * nyz));
+                float tau = 0.5f * (tau0 + sqrtf(tau0 * tau0 + 18.0f * sq2 * cs * cs * qn / rho));
+                float keep = 1.0f - 1.0f / tau;
+                float K = keep * 4.5f;
+                float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+                for (int i = 0; i < Q; i++) {
+                    float ex = (float)CX[i], ey = (float)CY[i], ez = (float)CZ[i];
+                    float qq = ((ex * ex - 1.0f / 3.0f) * nxx + (ey * ey - 1.0f / 3.0f) * nyy
+                                + (ez * ez - 1.0f / 3.0f) * nzz
+                                + 2.0f * (ex * ey * nxy + ex * ez * nxz + ey * ez * nyz));
+                    float po = fmaf(rho, E32(ex, ey, ez, ux, uy, uz, usq15), K * qq);
+                    enc16(f_out, IDX(i, x, y, z), st, po, SIG(sig, z, CX[i]), scale);
+                }
+            }
+    if (force_z) {
+        for (int z = 0; z < nz; z++) { force_z[2 * z] = 0.0; force_z[2 * z + 1] = 0.0; }
+        for (int x = 0; x < nx; x++)
+            for (int z = 0; z < nz; z++) {
+                force_z[2 * z] += part[((size_t)x * nz + z) * 2 + 0];
+                force_z[2 * z + 1] += part[((size_t)x * nz + z) * 2 + 1];
+            }
+        free(part);
+    }
+    return 0;
+}
+
+static int macro_h16(const void* f, int st, const uint8_t* flag, int nx, int ny, int nz, float scale,
+                     const float* sig, float* U, float* acc) {
+    const float inv_scale = 1.0f / scale;
+    const size_t N = (size_t)nx * ny * nz;
+    #pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                size_t c = FIDX(x, y, z);
+                float u[4];
+                if (flag[c] != 0) {
+                    u[0] = 1.0f; u[1] = 0.0f; u[2] = 0.0f; u[3] = 0.0f;
+                } else {
+                    float ph[Q], rho, mx, my, mz;
+                    for (int i = 0; i < Q; i++) ph[i] = dec16(f, IDX(i, x, y, z), st, SIG(sig, z, CX[i]), inv_scale);
+                    moments_h16(ph, &rho, &mx, &my, &mz);
+                    u[0] = rho; u[1] = mx / rho; u[2] = my / rho; u[3] = mz / rho;
+                }
+                for (int k = 0; k < 4; k++) {
+                    if (U) U[k * N + c] = u[k];
+                    if (acc) acc[k * N + c] = acc[k * N + c] + u[k];
+                }
             }
     return 0;
 }

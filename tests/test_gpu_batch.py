@@ -20,30 +20,30 @@ def wl(H, por, seed, steps):
     return workloads.porous_wall(H=H, porosity=por, seed=seed, n_steps=steps)
 
 
-def alone(w, dtype, variant, seed):
+def alone(w, dtype, variant, seed, shift=None):
     code = nat.DTYPES[dtype]
     old = nat.lib().sw_get_step_variant(code)
     nat.call("sw_set_step_variant", code, variant)
     try:
-        sim = Simulation(w.flag, w.uin, dtype=dtype, **w.sim_kwargs()).init(noise=0.02, seed=seed)
+        sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, **w.sim_kwargs()).init(noise=0.02, seed=seed)
         sim.step(w.n_steps)
         return sim.populations(), sim.force_rows()
     finally:
         nat.call("sw_set_step_variant", code, old)
 
 
-def lattice(w, dtype, seed, K):
-    return Lattice(w.flag, w.uin, dtype=dtype, ring=K, **w.sim_kwargs()).init(noise=0.02, seed=seed)
+def lattice(w, dtype, seed, K, shift=None):
+    return Lattice(w.flag, w.uin, dtype=dtype, ring=K, shift=shift, **w.sim_kwargs()).init(noise=0.02, seed=seed)
 
 
 @pytest.mark.parametrize("variant", [VARIANT, 8])
-@pytest.mark.parametrize("dtype", ["f32", "f16"])
-def test_alone_equals_mix(gpu, dtype, variant):
+@pytest.mark.parametrize("dtype,shift", [("f32", None), ("f16", "inlet"), ("f16", "weights"), ("bf16", "inlet")])
+def test_alone_equals_mix(gpu, dtype, shift, variant):
     K = 8
     ws = [wl(*c) for c in CASES]
-    ref = [alone(w, dtype, variant, c[2]) for w, c in zip(ws, CASES)]
+    ref = [alone(w, dtype, variant, c[2], shift) for w, c in zip(ws, CASES)]
     b = Batch(dtype, max_instances=8, K=K, variant=variant, graph=True)
-    lats = [lattice(w, dtype, c[2], K) for w, c in zip(ws, CASES)]
+    lats = [lattice(w, dtype, c[2], K, shift) for w, c in zip(ws, CASES)]
     for lat in lats:
         b.add(lat)
     b.run_to_completion()

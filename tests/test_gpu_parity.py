@@ -1,7 +1,8 @@
 """HIP kernels vs the CPU oracle, through the C ABI (needs an MI355X).
 
 Bar: FP32 / FP16 / BF16 storage bit-identical populations to the oracle (both compiled
-without FMA contraction, same operand order); force rows (FP64 sums in different fixed
+without FMA contraction, same operand order; the 16-bit forms with both storage offsets,
+the per-height inlet equilibrium and the weights); force rows (FP64 sums in different fixed
 orders) within 1e-12 relative; FP64 within 1e-14.  The workload-level bar (populations
 within 1e-5 relative after 1000 steps, mean Cd within 0.5 %) is asserted on the 160x80x40
 box-wall lattice in both y-boundary modes.
@@ -36,39 +37,74 @@ def fluid_mask(flag):
     return (flag == 0)[None]
 
 
-def run_both(flag, uin, steps, storage, ybc, gust=None, noise=0.02, seed=11, nu0=2e-4, x_sponge=None):
+def oracle_shift(uin, storage, shift):
+    """The oracle's offset table for a storage form (None: FP32 / FP64, or the weights)."""
+    if storage not in ("f16", "bf16") or shift == "weights":
+        return None
+    return co.shift_table(uin, shift or "inlet")
+
+
+def run_both(flag, uin, steps, storage, ybc, gust=None, noise=0.02, seed=11, nu0=2e-4, x_sponge=None, shift=None):
     nx = flag.shape[0]
     xs = nx - 10 if x_sponge is None else x_sponge
     sc = SCALE[storage]
     sim = Simulation(flag, uin, nu0=nu0, cs=0.16, x_sponge=xs, ybc=ybc, dtype=storage, scale=sc, gust=gust,
-                     n_steps=steps, ring=7)
+                     n_steps=steps, ring=7, shift=shift)
     sim.init(noise=noise, seed=seed)
-    f = co.init(flag, uin, storage, noise=noise, seed=seed, scale=sc)
+    sh = oracle_shift(uin, storage, shift)
+    f = co.init(flag, uin, storage, noise=noise, seed=seed, scale=sc, shift=sh)
     g0 = sim.populations()
     assert np.array_equal(np.where(fluid_mask(flag), g0, 0), np.where(fluid_mask(flag), f, 0)), "init differs"
     forces = []
     for t in range(steps):
         g = 1.0 if gust is None else float(gust[t])
-        f, F = co.step(f, flag, uin, nu0, 0.16, xs, 0.05, gust=g, ybc=ybc, storage=storage, scale=sc)
+        f, F = co.step(f, flag, uin, nu0, 0.16, xs, 0.05, gust=g, ybc=ybc, storage=storage, scale=sc, shift=sh)
         forces.append(F)
     sim.step(steps)
     return sim, f, np.array(forces)
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage,shift", [("f32", None), ("f16", "inlet"), ("f16", "weights"), ("bf16", "inlet"),
+                                           ("bf16", "weights")])
 @pytest.mark.parametrize("ybc", [0, 1])
-def test_step_bit_identical(gpu, storage, ybc):
+def test_step_bit_identical(gpu, storage, shift, ybc):
     flag, uin = small_case()
     gust = gust_table(make_gust(5), 40)
-    sim, f, F = run_both(flag, uin, 40, storage, ybc, gust=gust)
+    sim, f, F = run_both(flag, uin, 40, storage, ybc, gust=gust, shift=shift)
     g = sim.populations()
     fl = fluid_mask(flag)
     diff = np.where(fl, g != f, False)
-    assert not diff.any(), f"{diff.sum()} populations differ ({storage}, ybc={ybc})"
+    assert not diff.any(), f"{diff.sum()} populations differ ({storage}, {shift}, ybc={ybc})"
     Fg = sim.force_rows()
     assert Fg.shape == F.shape
     assert np.abs(F).max() > 0
     assert np.allclose(Fg, F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
+
+
+def test_shift_table_bit_identical(gpu):
+    """sw_shift_table (both kinds) equals the oracle's table bit for bit."""
+    from stonewall3d import _native as nat
+    uin = power_law_profile(37, 9)
+    d = torch.from_numpy(uin).cuda()
+    for kind in ("weights", "inlet"):
+        t = torch.empty((37, 4), dtype=torch.float32, device="cuda")
+        nat.call("sw_shift_table", d.data_ptr(), 37, nat.SHIFTS[kind], t.data_ptr(),
+                 torch.cuda.current_stream().cuda_stream)
+        assert np.array_equal(t.cpu().numpy().view(np.uint32), co.shift_table(uin, kind).view(np.uint32)), kind
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_macro_16bit_inlet_offsets_bit_identical(gpu, storage):
+    flag, uin = small_case()
+    sim, f, _ = run_both(flag, uin, 6, storage, 1, shift="inlet")
+    U = sim.macro()
+    Uo = co.macro(f, flag, storage, SCALE[storage], shift=co.shift_table(uin, "inlet"))
+    assert np.array_equal(U, Uo)
+    # the device checksum decodes with the same offsets
+    v = co.decode(f, storage, SCALE[storage], shift=co.shift_table(uin, "inlet")).astype(np.float64)
+    fl = fluid_mask(flag)
+    s0 = np.where(fl, v, 0).sum()
+    assert sim.checksum()[0] == pytest.approx(s0, rel=1e-12)
 
 
 def test_division_shortcut_rounds_like_ieee(gpu):

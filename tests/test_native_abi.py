@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_sizes():
     L = nat.lib()
-    assert L.sw_version() == 100
+    assert L.sw_version() == 110
     assert L.sw_rows_per_block() == 4
     assert L.sw_partials_per_step(320, 160) == 160 * 80 * 2
     assert L.sw_partials_per_step(321, 3) == 3 * 81 * 2
@@ -43,7 +43,8 @@ def test_version_and_sizes():
 def test_argument_errors_without_device():
     L = nat.lib()
     # null buffers
-    rc = L.sw_step(None, None, 0, None, None, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0, 1.0, None, 1, None)
+    rc = L.sw_step(None, None, 0, None, None, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0, 1.0, None, None, 1,
+                   None)
     assert rc == 1001
     assert b"non-null" in L.sw_last_error()
     fake = ctypes.c_void_p(0x1000)
@@ -53,20 +54,26 @@ def test_argument_errors_without_device():
     def step(**kw):
         a = dict(args, **kw)
         return L.sw_step(fake, ctypes.c_void_p(0x2000), a["dtype"], fake, fake, None, 0, None, a["nx"], a["ny"],
-                         a["nz"], 1e-3, 0.16, 8, 0.05, a["ybc"], a["scale"], None, 1, None)
+                         a["nz"], 1e-3, 0.16, 8, 0.05, a["ybc"], a["scale"], None, None, 1, None)
 
     assert step(nx=1) == 1001
     assert step(dtype=7) == 1001
     assert step(dtype=1, scale=3.0) == 1001          # 16-bit scale must be a power of two
+    assert step(dtype=1, scale=2.0 ** 21) == 1001    # ... in [2^-20, 2^20]
+    assert b"[2^-20, 2^20]" in L.sw_last_error()
     assert step(ybc=5) == 1001
     assert step(nx=2000, ny=2000, nz=100) == 1003
     assert b"113,025,455" in L.sw_last_error()
     # in-place is rejected (pull scheme needs two buffers)
-    assert L.sw_step(fake, fake, 0, fake, fake, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0, 1.0, None, 1,
+    assert L.sw_step(fake, fake, 0, fake, fake, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0, 1.0, None, None, 1,
                      None) == 1001
     # partials without a ring
     assert L.sw_step(fake, ctypes.c_void_p(0x2000), 0, fake, fake, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0,
-                     1.0, fake, 0, None) == 1001
+                     1.0, None, fake, 0, None) == 1001
+    # offset tables: unknown kind, missing inlet profile
+    assert L.sw_shift_table(fake, 8, 2, fake, None) == 1001
+    assert L.sw_shift_table(None, 8, 1, fake, None) == 1001
+    assert L.sw_shift_table(fake, 0, 1, fake, None) == 1001
     assert L.sw_force_reduce(fake, 4, 0, None, 5, 8, 8, fake, None) == 1001
     assert L.sw_to_ref_layout(fake, fake, 4, 19, 8, 8, 8, None) == 1001
     assert L.sw_to_ref_layout(fake, ctypes.c_void_p(0x2000), 3, 19, 8, 8, 8, None) == 1001
@@ -74,5 +81,5 @@ def test_argument_errors_without_device():
 
 def test_check_raises_with_message():
     with pytest.raises(nat.SwError) as e:
-        nat.call("sw_macro", None, 0, None, 8, 8, 8, 1.0, None, None, None)
+        nat.call("sw_macro", None, 0, None, 8, 8, 8, 1.0, None, None, None, None)
     assert e.value.code == 1001

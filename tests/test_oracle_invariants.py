@@ -321,3 +321,96 @@ def test_shifted_storage_rest_state(oracle_lib):
         rel = np.abs(np.where((flag == 0)[None], v - w, 0)) / w
         assert rel.max() < 1e-6
         assert np.all(force == 0.0)
+
+
+# ------------------------------------------------------------------ storage offsets (16-bit forms)
+def test_shift_table_closed_form(oracle_lib):
+    """sigma_c(z) = feq_i(1, (U, 0, 0)) / w_i for c_x = 0, +1, -1: 1 + 3cU + 4.5 c^2 U^2 - 1.5 U^2
+    to FP32 rounding (FP64 closed form); kind weights (U = 0) and uin[0] = 0 give 1."""
+    uin = profile(9, H=4.0, u=0.06)
+    w = co.shift_table(uin, "weights")
+    assert w.shape == (9, 4) and np.array_equal(w[:, :3], np.ones((9, 3), np.float32)) and np.all(w[:, 3] == 0)
+    s = co.shift_table(uin, "inlet")
+    u = uin.astype(np.float64)[:, None]
+    c = np.array([0.0, 1.0, -1.0])[None, :]
+    closed = 1 + 3 * c * u + 4.5 * c ** 2 * u ** 2 - 1.5 * u ** 2
+    assert np.allclose(s[:, :3], closed, rtol=2e-7, atol=0)
+    assert np.array_equal(s[0, :3], np.ones(3, np.float32))
+    # per direction: the equilibrium of the inlet velocity divided by the weight (FP64)
+    sig = co.sigma_per_direction(s, 9)
+    cu = 3 * nl.C[:, 0][:, None] * uin.astype(np.float64)[None, :]
+    feq_w = 1 + cu + 0.5 * cu * cu - 1.5 * uin.astype(np.float64)[None, :] ** 2
+    assert np.allclose(sig, feq_w, rtol=2e-7, atol=0)
+
+
+def _inlet_case(nx=14, ny=5, nz=8):
+    flag = small_flag(nx, ny, nz)
+    uin = profile(nz, H=3.0, u=0.06)
+    return flag, uin, co.shift_table(uin, "inlet")
+
+
+def test_inlet_shift_stores_inlet_state_exactly(oracle_lib):
+    """Without noise, every fluid cell starts at feq(1, (uin[z], 0, 0)): its stored value is
+    exactly zero with the inlet offsets, for both 16-bit forms, and decodes to the FP32
+    form's populations bit for bit."""
+    flag, uin, sh = _inlet_case()
+    fluid = (flag == 0)[None]
+    f32 = co.init(flag, uin, "f32")
+    for st, scale in (("f16", 1024.0), ("bf16", 1.0)):
+        f = co.init(flag, uin, st, scale=scale, shift=sh)
+        assert np.all(np.where(fluid, f, 0) == 0)
+        v = co.decode(f, st, scale, shift=sh)
+        assert np.array_equal(np.where(fluid, v, 0), np.where(fluid, f32, 0))
+
+
+def _half_ulp(q, st):
+    """Half a unit in the last place of stored 16-bit values (as float32 magnitudes)."""
+    if st == "f16":
+        h = q.view(np.float16)
+        return np.spacing(np.abs(h)).astype(np.float32) / 2
+    x = (q.astype(np.uint32) << 16).view(np.float32)
+    e = np.floor(np.log2(np.maximum(np.abs(x), np.float32(1e-30))))
+    return np.float32(2.0) ** (e - 7) / 2
+
+
+@pytest.mark.parametrize("st,scale", [("f16", 1024.0), ("bf16", 1.0)])
+@pytest.mark.parametrize("ybc", [0, 1])
+def test_16bit_step_matches_fp64_restatement(oracle_lib, st, scale, ybc):
+    """One 16-bit step (inlet offsets) from a noisy state equals the independent FP64 NumPy
+    step of the decoded inputs within the store rounding of each output (half an ulp of the
+    stored value) plus FP32 arithmetic (1.5e-7): every streamed link (offset of the source
+    height), the top and side mirrors, the bounce-backs and the inlet decode and collide
+    right.  Decoding any link with another height's offset misses by ~1e-4."""
+    flag, uin, sh = _inlet_case()
+    fluid = (flag == 0)[None]
+    f = co.init(flag, uin, st, noise=0.03, seed=5, scale=scale, shift=sh)
+    fin = co.decode(f, st, scale, shift=sh).astype(np.float64)
+    g, F = co.step(f, flag, uin, NU0, 0.16, 10, 0.05, gust=1.1, ybc=ybc, storage=st, scale=scale, shift=sh)
+    got = co.decode(g, st, scale, shift=sh).astype(np.float64)
+    ref, Fref = nl.step(fin, flag, uin, NU0, 0.16, 10, 0.05, gust=np.float32(1.1), ybc=ybc)
+    bound = nl.W.reshape(19, 1, 1, 1) * _half_ulp(g, st) / scale * 1.01 + 1.5e-7
+    err = np.where(fluid, np.abs(got - ref) - bound, -1)
+    assert err.max() <= 0, (st, float(err.max()))
+    assert np.abs(F - Fref).max() <= 1e-6 * np.abs(Fref).max()
+
+
+def test_inlet_shift_error_below_weight_shift_error(oracle_lib):
+    """Near the inlet state, storing f - feq(uin) loses much less than storing f - w: after
+    40 steps of a gently perturbed channel flow (ground only) the FP16 populations stay
+    several times closer to the FP32 run (RMS over the fluid populations)."""
+    flag = small_flag(16, 6, 8, wall=False)
+    uin = profile(8, H=3.0, u=0.06)
+    sh = co.shift_table(uin, "inlet")
+    fluid = (flag == 0)[None]
+    f32 = co.init(flag, uin, "f32", noise=0.01, seed=3)
+    fa = co.init(flag, uin, "f16", noise=0.01, seed=3, scale=1024.0, shift=sh)
+    fb = co.init(flag, uin, "f16", noise=0.01, seed=3, scale=1024.0)
+    for _ in range(40):
+        f32, _ = co.step(f32, flag, uin, NU0, 0.16, 12, 0.05, ybc=1)
+        fa, _ = co.step(fa, flag, uin, NU0, 0.16, 12, 0.05, ybc=1, storage="f16", scale=1024.0, shift=sh)
+        fb, _ = co.step(fb, flag, uin, NU0, 0.16, 12, 0.05, ybc=1, storage="f16", scale=1024.0)
+    n = fluid.sum() * 19
+    ea = np.sqrt((np.where(fluid, co.decode(fa, "f16", 1024.0, shift=sh) - f32, 0) ** 2).sum() / n)
+    eb = np.sqrt((np.where(fluid, co.decode(fb, "f16", 1024.0) - f32, 0) ** 2).sum() / n)
+    assert ea < 1e-6 and eb < 1e-5
+    assert ea < eb / 3, (ea, eb)
