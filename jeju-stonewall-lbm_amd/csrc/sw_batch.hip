@@ -55,7 +55,7 @@ __device__ __forceinline__ StepArgs instance_args(const sw_instance& I, int k) {
 }
 
 template <class S, class TL, bool BUF>
-__global__ __launch_bounds__(64 * TL::WPB, (TL::PAIR && sizeof(S) == 2) ? 6 : TL::OCC) void k_batch_step(const void* __restrict__ table, int k) {
+__global__ __launch_bounds__(64 * TL::WPB, TL::OCC) void k_batch_step(const void* __restrict__ table, int k) {
     const TableHeader* h = static_cast<const TableHeader*>(table);
     const sw_instance* e = table_entries(table);
     const int n = __builtin_amdgcn_readfirstlane(h->n), total = __builtin_amdgcn_readfirstlane(h->total_items);
