@@ -289,15 +289,16 @@ __host__ __device__ constexpr int xclass(int cx) { return cx == 0 ? 0 : (cx > 0 
 template <int I, class T>
 __device__ __forceinline__ T feq(T rho, T ux, T uy, T uz, T usq15);
 
-// feq_i(1, u) / w_i = ((1 + cu) + (0.5 cu) cu) - usq15 (lbm3d.py:35-36 order without the
-// weight): the equilibrium of the 16-bit forms, which hold phi_i = f_i / w_i
+// feq_i(1, u) / w_i = ((1 + (0.5 cu) cu) - usq15) + cu, cu = 3 c.u: the equilibrium of the
+// 16-bit forms, which hold phi_i = f_i / w_i (the terms of lbm3d.py:35-36 without the
+// weight, grouped so that a pair of opposite directions shares everything but the sign of cu)
 template <int I, class T>
 __device__ __forceinline__ T feq_norm(T ux, T uy, T uz, T usq15) {
     if constexpr (I == 0) {
         return T(1) - usq15;
     } else {
         const T cu = T(3) * cdotu<I>(ux, uy, uz);
-        return ((T(1) + cu) + (T(0.5) * cu) * cu) - usq15;
+        return ((T(1) + (T(0.5) * cu) * cu) - usq15) + cu;
     }
 }
 

@@ -119,24 +119,25 @@ __global__ void k_macro(const S* __restrict__ f, const uint32_t* __restrict__ li
         u[0] = 1.0f; u[1] = 0.0f; u[2] = 0.0f; u[3] = 0.0f;
     } else {
         T rho = T(0), mx = T(0), my = T(0), mz = T(0);
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            if constexpr (sizeof(S) == 2) {     // phi_i = f_i / w_i: the step's FMA chains
+        if constexpr (sizeof(S) == 2) {         // phi_i = f_i / w_i: the step's pair moments
+            T ph[Q];
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
                 const T sg = shift ? (T)shift[z * 4 + xclass(CX[i])] : T(1);
-                const T v = C::load(f, (size_t)i * N + self, sg, (T)inv_scale);
-                constexpr T w = wgt<T>(i);
-                rho = fma(w, v, rho);
-                if constexpr (CX[i] != 0) mx = fma(CX[i] > 0 ? w : -w, v, mx);
-                if constexpr (CY[i] != 0) my = fma(CY[i] > 0 ? w : -w, v, my);
-                if constexpr (CZ[i] != 0) mz = fma(CZ[i] > 0 ? w : -w, v, mz);
-            } else {
+                ph[i] = C::load(f, (size_t)i * N + self, sg, (T)inv_scale);
+            });
+            const Moments<T> m = moments_norm<T, false>(ph);
+            rho = m.rho; mx = m.mx; my = m.my; mz = m.mz;
+        } else {
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
                 const T v = C::load(f, (size_t)i * N + self, wgt<T>(i), (T)inv_scale);
                 rho = rho + v;
                 if constexpr (CX[i] != 0) mx = CX[i] > 0 ? mx + v : mx - v;
                 if constexpr (CY[i] != 0) my = CY[i] > 0 ? my + v : my - v;
                 if constexpr (CZ[i] != 0) mz = CZ[i] > 0 ? mz + v : mz - v;
-            }
-        });
+            });
+        }
         u[0] = (float)rho; u[1] = (float)(mx / rho); u[2] = (float)(my / rho); u[3] = (float)(mz / rho);
     }
 #pragma unroll

@@ -43,28 +43,58 @@ struct StepArgs {
 // results are stored through st(i, value).
 //   NORM = false: fl are the populations f_i; the reference's FP32 operation order.
 //   NORM = true : fl are the weight-normalised populations phi_i = f_i / w_i of the 16-bit
-//                 storage forms, and so are the results.  The moments are FMA chains in i
-//                 order, rho = sum w_i phi_i, m_a = sum (c_ia w_i) phi_i, Pi_ab = sum
-//                 (c_ia c_ib w_i) phi_i, each from +0; the stress, viscosity and relaxation
-//                 are those of NORM = false; phi_out = fma(rho, feq_i / w_i, (keep * 4.5) * qq).
+//                 storage forms, and so are the results.  The moments come from the sums
+//                 a_i = phi_i + phi_opp(i) and differences d_i = phi_i - phi_opp(i) of the
+//                 nine opposite pairs (moments_norm); the stress, viscosity and relaxation are
+//                 those of NORM = false; phi_out = fma(rho, feq_i / w_i, (keep * 4.5) * qq_i)
+//                 with feq_i / w_i = ((1 + (0.5 cu) cu) - 1.5 u.u) + cu (feq_norm).
+template <class T>
+struct Moments { T rho, mx, my, mz, pxx, pyy, pzz, pxy, pxz, pyz; };
+
+// Moments of nine opposite pairs (i, i+1), i odd, of weight-normalised populations:
+//   s = (a1 + a3) + a5, s' = ((((a7 + a9) + a11) + a13) + a15) + a17,
+//   rho = fma(w0, phi0, fma(w1, s, w2 s')),
+//   m_x = fma(w1, d1, w2 (((d7 + d9) + d11) + d13)),  m_y = fma(w1, d3, w2 (((d7 - d9) + d15) + d17)),
+//   m_z = fma(w1, d5, w2 (((d11 - d13) + d15) - d17)),
+//   Pi_xx = fma(w1, a1, w2 (((a7 + a9) + a11) + a13)),  Pi_yy = fma(w1, a3, w2 (((a7 + a9) + a15) + a17)),
+//   Pi_zz = fma(w1, a5, w2 (((a11 + a13) + a15) + a17)),
+//   Pi_xy = w2 (a7 - a9),  Pi_xz = w2 (a11 - a13),  Pi_yz = w2 (a15 - a17)
+// (w0, w1, w2 = 1/3, 1/18, 1/36; the pair leaders 1, 3, 5, 7, 9, 11, 13, 15, 17 have
+// c = +x, +y, +z, (1,1,0), (1,-1,0), (1,0,1), (1,0,-1), (0,1,1), (0,1,-1)).
+template <class T, bool STRESS>
+__device__ __forceinline__ Moments<T> moments_norm(const T (&fl)[Q]) {
+    constexpr T w0 = wgt<T>(0), w1 = wgt<T>(1), w2 = wgt<T>(7);
+    T ap[Q], dp[Q];
+    static_for<0, 9>([&](auto P) {
+        constexpr int i = 2 * P + 1;
+        ap[i] = fl[i] + fl[i + 1];
+        dp[i] = fl[i] - fl[i + 1];
+    });
+    Moments<T> m;
+    const T s1 = (ap[1] + ap[3]) + ap[5];
+    const T s2 = ((((ap[7] + ap[9]) + ap[11]) + ap[13]) + ap[15]) + ap[17];
+    m.rho = fma(w0, fl[0], fma(w1, s1, w2 * s2));
+    m.mx = fma(w1, dp[1], w2 * (((dp[7] + dp[9]) + dp[11]) + dp[13]));
+    m.my = fma(w1, dp[3], w2 * (((dp[7] - dp[9]) + dp[15]) + dp[17]));
+    m.mz = fma(w1, dp[5], w2 * (((dp[11] - dp[13]) + dp[15]) - dp[17]));
+    if constexpr (STRESS) {
+        m.pxx = fma(w1, ap[1], w2 * (((ap[7] + ap[9]) + ap[11]) + ap[13]));
+        m.pyy = fma(w1, ap[3], w2 * (((ap[7] + ap[9]) + ap[15]) + ap[17]));
+        m.pzz = fma(w1, ap[5], w2 * (((ap[11] + ap[13]) + ap[15]) + ap[17]));
+        m.pxy = w2 * (ap[7] - ap[9]);
+        m.pxz = w2 * (ap[11] - ap[13]);
+        m.pyz = w2 * (ap[15] - ap[17]);
+    }
+    return m;
+}
+
 template <bool NORM, class T, class Store>
 __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const StepArgs& a, T smag, Store&& st) {
     T rho, mx, my, mz, pxx, pyy, pzz, pxy, pxz, pyz;
     if constexpr (NORM) {
-        rho = mx = my = mz = pxx = pyy = pzz = pxy = pxz = pyz = T(0);
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr int cx = CX[i], cy = CY[i], cz = CZ[i];
-            constexpr T w = wgt<T>(i);
-            const T v = fl[i];
-            rho = fma(w, v, rho);
-            if constexpr (cx != 0) { mx = fma(cx > 0 ? w : -w, v, mx); pxx = fma(w, v, pxx); }
-            if constexpr (cy != 0) { my = fma(cy > 0 ? w : -w, v, my); pyy = fma(w, v, pyy); }
-            if constexpr (cz != 0) { mz = fma(cz > 0 ? w : -w, v, mz); pzz = fma(w, v, pzz); }
-            if constexpr (cx * cy != 0) pxy = fma(cx * cy > 0 ? w : -w, v, pxy);
-            if constexpr (cx * cz != 0) pxz = fma(cx * cz > 0 ? w : -w, v, pxz);
-            if constexpr (cy * cz != 0) pyz = fma(cy * cz > 0 ? w : -w, v, pyz);
-        });
+        const Moments<T> m = moments_norm<T, true>(fl);
+        rho = m.rho; mx = m.mx; my = m.my; mz = m.mz;
+        pxx = m.pxx; pyy = m.pyy; pzz = m.pzz; pxy = m.pxy; pxz = m.pxz; pyz = m.pyz;
     } else {
         // ---- moments (:103-127), sequential in i; zero-coefficient terms dropped
         rho = fl[0];
@@ -136,8 +166,9 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
         const T h = (T(0.5) * cu) * cu;
         if constexpr (NORM) {
             const T kq = K * qq;
-            st(std::integral_constant<int, i>{}, fma(rho, ((T(1) + cu) + h) - usq15, kq));
-            st(std::integral_constant<int, j>{}, fma(rho, ((T(1) + (-cu)) + h) - usq15, kq));
+            const T base = (T(1) + h) - usq15;                 // feq_norm of the pair: base +- cu
+            st(std::integral_constant<int, i>{}, fma(rho, base + cu, kq));
+            st(std::integral_constant<int, j>{}, fma(rho, base - cu, kq));
         } else {
             const T neq = ((keep * w) * T(4.5)) * qq;
             const T wr = w * rho;

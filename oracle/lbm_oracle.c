@@ -540,12 +540,11 @@ int oracle_qcoarse(const float* U, const uint8_t* flag, int nx, int ny, int nz, 
  *
  * The step's FP32 arithmetic on phi, this build's own definition for these forms:
  *   phi_i    = fmaf(d_i, 1/scale, sigma(z_src, c_x of the stored direction))  (one rounding)
- *   rho      = fmaf chain over i = 0..18 of w_i * phi_i, starting from +0
- *   m_a      = fmaf chain over i with c_ia != 0 of (c_ia w_i) * phi_i, from +0
- *   Pi_ab    = fmaf chain over i with c_ia c_ib != 0 of (c_ia c_ib w_i) * phi_i, from +0
+ *   moments  from the nine opposite pairs (i, i+1), i odd: a_i = phi_i + phi_i+1,
+ *            d_i = phi_i - phi_i+1 (moments_pairs below gives the exact grouping)
  *   u, N, nu, tau, keep: exactly as the FP32 step (lbm3d.py:128-147)
- *   phi_out  = fmaf(rho, E_i, (keep * 4.5) * qq_i),  E_i = ((1 + cu) + (0.5 cu) cu) - 1.5 u.u
- *              (feq_i / w_i, lbm3d.py:35-36 order; qq_i as lbm3d.py:150-154)
+ *   phi_out  = fmaf(rho, E_i, (keep * 4.5) * qq_i),  E_i = ((1 + (0.5 cu) cu) - 1.5 u.u) + cu
+ *              (feq_i / w_i; qq_i as lbm3d.py:150-154)
  *   inlet    = E_i at rho = 1, u = (uin[z] gust, 0, 0)
  *   wall force per stone link: fr2 = (2 w_i) * (phi_bb - 1) = 2 (f - w_i) (lbm3d.py:92-94)
  */
@@ -561,10 +560,11 @@ static inline void enc16(void* f, size_t idx, int st, float v, float s, float sc
     float d = (v - s) * scale;
     ((uint16_t*)f)[idx] = (st == ST_F16) ? oracle_f32_to_f16(d) : oracle_f32_to_bf16(d);
 }
-/* feq_i / w_i of lbm3d.py:35-36 (rho = 1), usq15 = 1.5 * ((ux ux + uy uy) + uz uz) */
+/* feq_i / w_i (rho = 1) grouped as ((1 + (0.5 cu) cu) - usq15) + cu, cu = 3 c.u,
+ * usq15 = 1.5 * ((ux ux + uy uy) + uz uz) */
 static inline float E32(float cx, float cy, float cz, float ux, float uy, float uz, float usq15) {
     float cu = 3.0f * (cx * ux + cy * uy + cz * uz);
-    return 1.0f + cu + 0.5f * cu * cu - usq15;
+    return ((1.0f + 0.5f * cu * cu) - usq15) + cu;
 }
 
 /* sigma table [nz][4] of the 16-bit forms: kind 1 = U(z) = uin[z] (the inlet equilibrium),
@@ -597,17 +597,25 @@ static int init_h16(void* f, int st, const uint8_t* flag, const float* uin, int 
     return 0;
 }
 
-/* rho and momentum of 19 normalised populations (the step's chains) */
-static inline void moments_h16(const float* ph, float* rho, float* mx, float* my, float* mz) {
-    float r = 0.0f, a = 0.0f, b = 0.0f, c = 0.0f;
-    for (int i = 0; i < Q; i++) {
-        float w = W32(i);
-        r = fmaf(w, ph[i], r);
-        if (CX[i]) a = fmaf((float)CX[i] * w, ph[i], a);
-        if (CY[i]) b = fmaf((float)CY[i] * w, ph[i], b);
-        if (CZ[i]) c = fmaf((float)CZ[i] * w, ph[i], c);
-    }
-    *rho = r; *mx = a; *my = b; *mz = c;
+/* Moments of 19 normalised populations from the opposite pairs (i, i+1), i odd (pair leaders
+ * 1, 3, 5, 7, 9, 11, 13, 15, 17 have c = +x, +y, +z, (1,1,0), (1,-1,0), (1,0,1), (1,0,-1),
+ * (0,1,1), (0,1,-1)):  m[0..9] = rho, mx, my, mz, pxx, pyy, pzz, pxy, pxz, pyz. */
+static inline void moments_pairs(const float* ph, float* m) {
+    const float w0 = W32(0), w1 = W32(1), w2 = W32(7);
+    float a[Q], d[Q];
+    for (int i = 1; i < Q; i += 2) { a[i] = ph[i] + ph[i + 1]; d[i] = ph[i] - ph[i + 1]; }
+    float s1 = (a[1] + a[3]) + a[5];
+    float s2 = ((((a[7] + a[9]) + a[11]) + a[13]) + a[15]) + a[17];
+    m[0] = fmaf(w0, ph[0], fmaf(w1, s1, w2 * s2));
+    m[1] = fmaf(w1, d[1], w2 * (((d[7] + d[9]) + d[11]) + d[13]));
+    m[2] = fmaf(w1, d[3], w2 * (((d[7] - d[9]) + d[15]) + d[17]));
+    m[3] = fmaf(w1, d[5], w2 * (((d[11] - d[13]) + d[15]) - d[17]));
+    m[4] = fmaf(w1, a[1], w2 * (((a[7] + a[9]) + a[11]) + a[13]));
+    m[5] = fmaf(w1, a[3], w2 * (((a[7] + a[9]) + a[15]) + a[17]));
+    m[6] = fmaf(w1, a[5], w2 * (((a[11] + a[13]) + a[15]) + a[17]));
+    m[7] = w2 * (a[7] - a[9]);
+    m[8] = w2 * (a[11] - a[13]);
+    m[9] = w2 * (a[15] - a[17]);
 }
 
 static int step_h16(const void* f_in, void* f_out, int st, const uint8_t* flag, const float* uin, float gust,
@@ -659,18 +667,10 @@ static int step_h16(const void* f_in, void* f_out, int st, const uint8_t* flag, 
                     part[((size_t)x * nz + z) * 2 + 0] += (double)fxw;
                     part[((size_t)x * nz + z) * 2 + 1] += (double)fzw;
                 }
-                float rho, mx, my, mz;
-                moments_h16(ph, &rho, &mx, &my, &mz);
-                float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
-                for (int i = 0; i < Q; i++) {
-                    float w = W32(i);
-                    if (CX[i]) pxx = fmaf(w, ph[i], pxx);
-                    if (CY[i]) pyy = fmaf(w, ph[i], pyy);
-                    if (CZ[i]) pzz = fmaf(w, ph[i], pzz);
-                    if (CX[i] * CY[i] != 0) pxy = fmaf((float)(CX[i] * CY[i]) * w, ph[i], pxy);
-                    if (CX[i] * CZ[i] != 0) pxz = fmaf((float)(CX[i] * CZ[i]) * w, ph[i], pxz);
-                    if (CY[i] * CZ[i] != 0) pyz = fmaf((float)(CY[i] * CZ[i]) * w, ph[i], pyz);
-                }
+                float mo[10];
+                moments_pairs(ph, mo);
+                float rho = mo[0], mx = mo[1], my = mo[2], mz = mo[3];
+                float pxx = mo[4], pyy = mo[5], pzz = mo[6], pxy = mo[7], pxz = mo[8], pyz = mo[9];
                 float ux = mx / rho, uy = my / rho, uz = mz / rho;
                 float third = rho / 3.0f;
                 float nxx = pxx - third - rho * ux * ux;
@@ -724,10 +724,10 @@ static int macro_h16(const void* f, int st, const uint8_t* flag, int nx, int ny,
                 if (flag[c] != 0) {
                     u[0] = 1.0f; u[1] = 0.0f; u[2] = 0.0f; u[3] = 0.0f;
                 } else {
-                    float ph[Q], rho, mx, my, mz;
+                    float ph[Q], mo[10];
                     for (int i = 0; i < Q; i++) ph[i] = dec16(f, IDX(i, x, y, z), st, SIG(sig, z, CX[i]), inv_scale);
-                    moments_h16(ph, &rho, &mx, &my, &mz);
-                    u[0] = rho; u[1] = mx / rho; u[2] = my / rho; u[3] = mz / rho;
+                    moments_pairs(ph, mo);
+                    u[0] = mo[0]; u[1] = mo[1] / mo[0]; u[2] = mo[2] / mo[0]; u[3] = mo[3] / mo[0];
                 }
                 for (int k = 0; k < 4; k++) {
                     if (U) U[k * N + c] = u[k];
