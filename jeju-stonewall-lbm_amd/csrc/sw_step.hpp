@@ -29,7 +29,7 @@ struct StepArgs {
     const float* gust;       // [t] gust factor per step, or null (factor 1)
     double* partials;        // [ring][nz][nyb][2], or null
     const int* t_dev;        // optional device step counter added to t
-    const float* shift;      // 16-bit storage: [nz][19] stored offset s_i(z), or null (s_i = w_i)
+    const float* shift;      // 16-bit storage: [nz][4] offsets sigma (sw_shift_table), or null (sigma = 1)
     int t;
     int ring;
     int nx, ny, nz, nyb;
