@@ -390,6 +390,17 @@ __device__ __forceinline__ T wave_uniform(const T& v) {
     return r;
 }
 
+// A copy of a wave-uniform float held in a vector register.  A gfx9 VALU instruction reads
+// at most one scalar register, and the 16-bit decode's multiply-add takes two wave-uniform
+// operands (the inverse scale and a per-row offset): with the inverse scale in a vector
+// register the offset is read straight from its scalar register instead of being copied to
+// a vector register before each decode.
+__device__ __forceinline__ float vgpr_copy(float s) {
+    float v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+
 // ------------------------------------------------------------- wave reductions
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

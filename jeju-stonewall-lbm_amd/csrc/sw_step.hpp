@@ -46,8 +46,8 @@ struct StepArgs {
 //                 storage forms, and so are the results.  The moments come from the sums
 //                 a_i = phi_i + phi_opp(i) and differences d_i = phi_i - phi_opp(i) of the
 //                 nine opposite pairs (moments_norm); the stress, viscosity and relaxation are
-//                 those of NORM = false; phi_out = fma(rho, feq_i / w_i, (keep * 4.5) * qq_i)
-//                 with feq_i / w_i = ((1 + (0.5 cu) cu) - 1.5 u.u) + cu (feq_norm).
+//                 those of NORM = false; phi_out = rho feq_i / w_i + (keep * 4.5) qq_i, formed
+//                 from terms shared across directions (see the NORM branch below).
 template <class T>
 struct Moments { T rho, mx, my, mz, pxx, pyy, pzz, pxy, pxz, pyz; };
 
@@ -146,12 +146,48 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
     constexpr T third_c = T(1) / T(3);
     constexpr T A1 = T(1) - third_c;      // (c*c - 1/3) for |c| = 1
     constexpr T A0 = T(0) - third_c;      // (c*c - 1/3) for c = 0
-    const T K = keep * T(4.5);            // NORM: (keep * 4.5) * qq is the normalised neq
+    if constexpr (NORM) {
+        const T K = keep * T(4.5);        // (keep * 4.5) * qq is the normalised neq
+        // Normalised output phi_out_i = rho E_i + K qq_i from terms shared across directions.
+        // With the trace term Kt = K (nxx + nyy + nzz) / 3, K qq_i is K n_aa - Kt for an axis
+        // direction a, (K n_aa + K n_bb - Kt) +- 2 K n_ab for a diagonal one in the ab plane,
+        // and -Kt at rest; rho E_i = rho (1 - 1.5 u.u) + (0.5 rho cu) cu +- rho cu.  Per
+        // opposite pair: R = rho (1 - 1.5 u.u) + K qq, B = fma(0.5 rho cu, cu, R),
+        // phi_out = fma(+-rho, cu, B).  75 operations for the 19 outputs.
+        static_assert(CX[1] == 1 && CY[3] == 1 && CZ[5] == 1, "axis pair leaders");
+        static_assert(CX[7] == 1 && CY[7] == 1 && CX[9] == 1 && CY[9] == -1, "xy pair leaders");
+        static_assert(CX[11] == 1 && CZ[11] == 1 && CX[13] == 1 && CZ[13] == -1, "xz pair leaders");
+        static_assert(CY[15] == 1 && CZ[15] == 1 && CY[17] == 1 && CZ[17] == -1, "yz pair leaders");
+        const T u3x = T(3) * ux, u3y = T(3) * uy, u3z = T(3) * uz;
+        const T kxx = K * nxx, kyy = K * nyy, kzz = K * nzz;
+        const T k2 = T(2) * K;
+        const T kxy = k2 * nxy, kxz = k2 * nxz, kyz = k2 * nyz;
+        const T sxy = kxx + kyy, sxz = kxx + kzz, syz = kyy + kzz;
+        const T kt = (sxy + kzz) * third_c;
+        const T r0 = rho * (T(1) - usq15) - kt;
+        const T rh = T(0.5) * rho;
+        st(std::integral_constant<int, 0>{}, r0);
+        auto pair = [&](auto I, T cu, T R) {
+            const T B = fma(rh * cu, cu, R);
+            st(I, fma(rho, cu, B));
+            st(std::integral_constant<int, decltype(I)::value + 1>{}, fma(-rho, cu, B));
+        };
+        pair(std::integral_constant<int, 1>{}, u3x, r0 + kxx);
+        pair(std::integral_constant<int, 3>{}, u3y, r0 + kyy);
+        pair(std::integral_constant<int, 5>{}, u3z, r0 + kzz);
+        const T gxy = r0 + sxy, gxz = r0 + sxz, gyz = r0 + syz;
+        pair(std::integral_constant<int, 7>{}, u3x + u3y, gxy + kxy);
+        pair(std::integral_constant<int, 9>{}, u3x - u3y, gxy - kxy);
+        pair(std::integral_constant<int, 11>{}, u3x + u3z, gxz + kxz);
+        pair(std::integral_constant<int, 13>{}, u3x - u3z, gxz - kxz);
+        pair(std::integral_constant<int, 15>{}, u3y + u3z, gyz + kyz);
+        pair(std::integral_constant<int, 17>{}, u3y - u3z, gyz - kyz);
+        return;
+    }
     {
         constexpr T w = wgt<T>(0);
         const T qq = (A0 * nxx + A0 * nyy) + A0 * nzz;
-        if constexpr (NORM) st(std::integral_constant<int, 0>{}, fma(rho, T(1) - usq15, K * qq));
-        else st(std::integral_constant<int, 0>{}, feq<0, T>(rho, ux, uy, uz, usq15) + ((keep * w) * T(4.5)) * qq);
+        st(std::integral_constant<int, 0>{}, feq<0, T>(rho, ux, uy, uz, usq15) + ((keep * w) * T(4.5)) * qq);
     }
     static_for<0, 9>([&](auto P) {
         constexpr int i = 2 * P + 1, j = i + 1;
@@ -164,17 +200,10 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
         if constexpr (cy * cz != 0) qq = qq + T(2) * (cy * cz > 0 ? nyz : -nyz);
         const T cu = T(3) * cdotu<i>(ux, uy, uz);
         const T h = (T(0.5) * cu) * cu;
-        if constexpr (NORM) {
-            const T kq = K * qq;
-            const T base = (T(1) + h) - usq15;                 // feq_norm of the pair: base +- cu
-            st(std::integral_constant<int, i>{}, fma(rho, base + cu, kq));
-            st(std::integral_constant<int, j>{}, fma(rho, base - cu, kq));
-        } else {
-            const T neq = ((keep * w) * T(4.5)) * qq;
-            const T wr = w * rho;
-            st(std::integral_constant<int, i>{}, wr * (((T(1) + cu) + h) - usq15) + neq);
-            st(std::integral_constant<int, j>{}, wr * (((T(1) + (-cu)) + h) - usq15) + neq);
-        }
+        const T neq = ((keep * w) * T(4.5)) * qq;
+        const T wr = w * rho;
+        st(std::integral_constant<int, i>{}, wr * (((T(1) + cu) + h) - usq15) + neq);
+        st(std::integral_constant<int, j>{}, wr * (((T(1) + (-cu)) + h) - usq15) + neq);
     });
 }
 
@@ -251,7 +280,10 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     const uint32_t rowid = (uint32_t)z * ny + y;
     const uint32_t selfrow = rowid * nx;
     const size_t selfb = (size_t)selfrow * ES;
-    const T inv_scale = (T)a.inv_scale, scale = (T)a.scale;
+    T inv_scale = (T)a.inv_scale, scale = (T)a.scale;
+    // 16-bit decodes fma(h, inv_scale, sigma): inv_scale in a vector register, sigma scalar.
+    // (A vector copy of the encode scale as well costs a VGPR: 73, one wavefront per SIMD less.)
+    if constexpr (ES == 2) inv_scale = vgpr_copy(a.inv_scale);
     const int kscale = (int)((__builtin_bit_cast(uint32_t, a.scale) >> 23) & 0xFFu) - 127;
     // (18*sqrt(2))*cs*cs evaluated left to right as lbm3d.py:146 does
     const T cs = (T)a.cs;

@@ -543,8 +543,13 @@ int oracle_qcoarse(const float* U, const uint8_t* flag, int nx, int ny, int nz, 
  *   moments  from the nine opposite pairs (i, i+1), i odd: a_i = phi_i + phi_i+1,
  *            d_i = phi_i - phi_i+1 (moments_pairs below gives the exact grouping)
  *   u, N, nu, tau, keep: exactly as the FP32 step (lbm3d.py:128-147)
- *   phi_out  = fmaf(rho, E_i, (keep * 4.5) * qq_i),  E_i = ((1 + (0.5 cu) cu) - 1.5 u.u) + cu
- *              (feq_i / w_i; qq_i as lbm3d.py:150-154)
+ *   phi_out  = rho E_i + K qq_i, K = keep * 4.5, E_i = feq_i / w_i (lbm3d.py:35-36 at w = 1),
+ *              qq_i as lbm3d.py:150-154, grouped through terms shared by the directions
+ *              (collide_pairs below): kt = ((K nxx + K nyy) + K nzz) / 3,
+ *              r0 = rho (1 - 1.5 u.u) - kt (the rest direction); per opposite pair (i, i+1)
+ *              R = r0 + K n_aa (axis a) or (r0 + (K n_aa + K n_bb)) +- (2K) n_ab (diagonal in
+ *              the ab plane), cu = 3 u_a or 3 u_a +- 3 u_b, B = fmaf(0.5 rho cu, cu, R),
+ *              phi_out_i = fmaf(rho, cu, B), phi_out_i+1 = fmaf(-rho, cu, B)
  *   inlet    = E_i at rho = 1, u = (uin[z] gust, 0, 0)
  *   wall force per stone link: fr2 = (2 w_i) * (phi_bb - 1) = 2 (f - w_i) (lbm3d.py:92-94)
  */
@@ -616,6 +621,30 @@ static inline void moments_pairs(const float* ph, float* m) {
     m[7] = w2 * (a[7] - a[9]);
     m[8] = w2 * (a[11] - a[13]);
     m[9] = w2 * (a[15] - a[17]);
+}
+
+/* Normalised collision outputs out[0..18] = rho E_i + K qq_i of one cell (the grouping in the
+ * comment above); pair leaders as in moments_pairs. */
+static inline void collide_pairs(float rho, float ux, float uy, float uz, float nxx, float nyy, float nzz,
+                                 float nxy, float nxz, float nyz, float K, float usq15, float* out) {
+    const float u3x = 3.0f * ux, u3y = 3.0f * uy, u3z = 3.0f * uz;
+    const float kxx = K * nxx, kyy = K * nyy, kzz = K * nzz;
+    const float k2 = 2.0f * K;
+    const float kxy = k2 * nxy, kxz = k2 * nxz, kyz = k2 * nyz;
+    const float sxy = kxx + kyy, sxz = kxx + kzz, syz = kyy + kzz;
+    const float kt = (sxy + kzz) * (1.0f / 3.0f);
+    const float r0 = rho * (1.0f - usq15) - kt;
+    const float rh = 0.5f * rho;
+    const float gxy = r0 + sxy, gxz = r0 + sxz, gyz = r0 + syz;
+    const float cu[9] = {u3x, u3y, u3z, u3x + u3y, u3x - u3y, u3x + u3z, u3x - u3z, u3y + u3z, u3y - u3z};
+    const float R[9] = {r0 + kxx, r0 + kyy, r0 + kzz, gxy + kxy, gxy - kxy, gxz + kxz, gxz - kxz,
+                        gyz + kyz, gyz - kyz};
+    out[0] = r0;
+    for (int p = 0; p < 9; p++) {
+        const float B = fmaf(rh * cu[p], cu[p], R[p]);
+        out[2 * p + 1] = fmaf(rho, cu[p], B);
+        out[2 * p + 2] = fmaf(-rho, cu[p], B);
+    }
 }
 
 static int step_h16(const void* f_in, void* f_out, int st, const uint8_t* flag, const float* uin, float gust,
@@ -690,14 +719,9 @@ static int step_h16(const void* f_in, void* f_out, int st, const uint8_t* flag, 
                 float keep = 1.0f - 1.0f / tau;
                 float K = keep * 4.5f;
                 float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
-                for (int i = 0; i < Q; i++) {
-                    float ex = (float)CX[i], ey = (float)CY[i], ez = (float)CZ[i];
-                    float qq = ((ex * ex - 1.0f / 3.0f) * nxx + (ey * ey - 1.0f / 3.0f) * nyy
-                                + (ez * ez - 1.0f / 3.0f) * nzz
-                                + 2.0f * (ex * ey * nxy + ex * ez * nxz + ey * ez * nyz));
-                    float po = fmaf(rho, E32(ex, ey, ez, ux, uy, uz, usq15), K * qq);
-                    enc16(f_out, IDX(i, x, y, z), st, po, SIG(sig, z, CX[i]), scale);
-                }
+                float po[Q];
+                collide_pairs(rho, ux, uy, uz, nxx, nyy, nzz, nxy, nxz, nyz, K, usq15, po);
+                for (int i = 0; i < Q; i++) enc16(f_out, IDX(i, x, y, z), st, po[i], SIG(sig, z, CX[i]), scale);
             }
     if (force_z) {
         for (int z = 0; z < nz; z++) { force_z[2 * z] = 0.0; force_z[2 * z + 1] = 0.0; }
