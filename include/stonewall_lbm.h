@@ -25,8 +25,9 @@
  *     (phi_i - sigma) * scale, scale a power of two in [2^-20, 2^20], sigma = sigma_c(z) of
  *     the direction's c_x class c (0, +1, -1) at the cell's height, from a `shift` table
  *     [nz][4] float (device; sw_shift_table) or 1 when `shift` is NULL.  The collision runs
- *     on phi in FP32 (moments from opposite-pair sums and differences; the rest as the
- *     reference).
+ *     on phi in FP32 (moments from opposite-pair sums and differences; velocity, stress
+ *     and relaxation time as the reference; outputs rho feq_i / w_i + (keep * 4.5) qq_i
+ *     from terms shared across directions).
  *     The same table must be passed to every call on one population state.
  *   - A lattice holds at most 113,025,455 cells.
  */
