@@ -198,16 +198,18 @@ template <class S> struct PopMem<S, true> {
     __amdgpu_buffer_rsrc_t rin, rout;
     __device__ __forceinline__ PopMem(const void* fin, void* fout) : rin(make_rsrc(fin)), rout(make_rsrc(fout)) {}
     // 32-bit access (two adjacent 16-bit populations, or one 32-bit edge value)
+    template <bool NT = false>
     __device__ __forceinline__ uint32_t ld32(size_t row_bytes, uint32_t lane_bytes) const {
-        return buf_load<uint32_t>(rin, lane_bytes, (uint32_t)row_bytes);
+        return buf_load<uint32_t, NT ? 2 : 0>(rin, lane_bytes, (uint32_t)row_bytes);
     }
     template <bool NT = false>
     __device__ __forceinline__ void st32(size_t row_bytes, uint32_t lane_bytes, uint32_t v) const {
         buf_store<uint32_t, NT ? 2 : 0>(v, rout, lane_bytes, (uint32_t)row_bytes);
     }
     // 64-bit access (two adjacent 32-bit populations)
+    template <bool NT = false>
     __device__ __forceinline__ uint64_t ld64(size_t row_bytes, uint32_t lane_bytes) const {
-        return buf_load<uint64_t>(rin, lane_bytes, (uint32_t)row_bytes);
+        return buf_load<uint64_t, NT ? 2 : 0>(rin, lane_bytes, (uint32_t)row_bytes);
     }
     // per-lane byte offset of any population (all below 4 GiB here)
     __device__ __forceinline__ B ld_at(size_t bytes) const { return buf_load<B>(rin, (uint32_t)bytes, 0u); }
@@ -230,6 +232,7 @@ template <class S> struct PopMem<S, false> {
     char* out;
     __device__ __forceinline__ PopMem(const void* fin, void* fout)
         : in(static_cast<const char*>(fin)), out(static_cast<char*>(fout)) {}
+    template <bool NT = false>
     __device__ __forceinline__ uint32_t ld32(size_t row_bytes, uint32_t lane_bytes) const {
         return *reinterpret_cast<const uint32_t*>(in + row_bytes + lane_bytes);
     }
@@ -238,6 +241,7 @@ template <class S> struct PopMem<S, false> {
         uint32_t* p = reinterpret_cast<uint32_t*>(out + row_bytes + lane_bytes);
         if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
     }
+    template <bool NT = false>
     __device__ __forceinline__ uint64_t ld64(size_t row_bytes, uint32_t lane_bytes) const {
         return *reinterpret_cast<const uint64_t*>(in + row_bytes + lane_bytes);
     }

@@ -298,7 +298,7 @@ using namespace sw;
 
 // Step variants (the autotuner's choices, per dtype): the tile list of sw_step.hpp.
 // Lattices whose byte offsets exceed 32 bits use 64-bit addressing with nontemporal stores.
-static int g_step_variant[4] = {9, 8, 8, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
+static int g_step_variant[4] = {9, 9, 9, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
 
 template <class S, class TL, bool BUF>
 static void launch_step_v(const StepArgs& a, hipStream_t s) {

@@ -566,7 +566,12 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                 }
             }
         };
-        auto issue = [&](uint32_t c, uint32_t (&cur)[Q], uint32_t (&edge)[Q]) {
+        // The left-edge word of a c_x = +1 link (cells x0-2, x0-1) is lane 63's word of the
+        // same link in the previous chunk of this sweep: after a fast pair chunk it is carried
+        // over in a scalar register (carry[]) instead of being loaded again, a load whose
+        // 64-byte sector has usually left the L2 by then.
+        auto issue = [&](uint32_t c, uint32_t (&cur)[Q], uint32_t (&edge)[Q], bool carried,
+                         const uint32_t (&carry)[Q]) {
             const uint32_t x0 = c * 128;
             const uint32_t lb = min(x0 + 2 * lane, nx - 2) * 2;          // lanes past the row end read in-row
             const uint32_t ledge = (x0 >= 2 ? x0 - 2 : 0) * 2;            // dword holding cell x0-1
@@ -574,12 +579,21 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
                 const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
-                cur[i] = mem.ld32(row, lb);
-                if constexpr (CX[i] > 0) edge[i] = mem.ld32(row, ledge);
+                cur[i] = mem.template ld32<TL::NTL>(row, lb);
                 if constexpr (CX[i] < 0) edge[i] = mem.ld32(row, redge);
             });
+            if (carried) {
+                static_for<0, Q>([&](auto I) {
+                    if constexpr (CX[I] > 0) edge[I] = carry[I];
+                });
+            } else {
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    if constexpr (CX[i] > 0) edge[i] = mem.ld32(srow[i] + ES, ledge);
+                });
+            }
         };
-        auto finish = [&](uint32_t c, const uint32_t (&cur)[Q], const uint32_t (&edge)[Q]) {
+        auto finish = [&](uint32_t c, const uint32_t (&cur)[Q], const uint32_t (&edge)[Q], uint32_t (&carry)[Q]) {
             const uint32_t x0 = c * 128;
             const uint32_t xa = x0 + 2 * lane;
             const uint32_t xl = min(xa, nx - 2);
@@ -590,6 +604,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                     raw[i] = cur[i];
                 } else if constexpr (CX[i] > 0) {                        // cells (x-1, x)
                     raw[i] = mid16(cur[i], wave_shr1(cur[i], edge[i]));
+                    carry[i] = __builtin_amdgcn_readlane(cur[i], 63);   // cells x0+126, x0+127
                 } else {                                                 // cells (x+1, x+2)
                     uint32_t nxt = wave_shl1(cur[i], edge[i]);
                     if (xl + 2 >= nx) nxt = cur[i] >> 16;               // outlet: cell nx-1 pulls itself
@@ -630,15 +645,19 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                 });
             }
         };
+        uint32_t carry[Q] = {};
+        bool carried = false;
 #pragma clang loop unroll(disable)
         for (uint32_t c = pbeg; c < pend; c++) {
             if (pair_special(c)) {
                 special_pair_chunk(c);
+                carried = false;
                 continue;
             }
             uint32_t cur[Q], edge[Q];
-            issue(c, cur, edge);
-            finish(c, cur, edge);
+            issue(c, cur, edge, carried, carry);
+            finish(c, cur, edge, carry);
+            carried = true;
         }
       }
     } else if constexpr (TL::PAIR && ES == 4) {
@@ -648,12 +667,15 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         // 128-cell chunks; lane L owns cells xa = x0 + 2L and xa + 1 (nx is even).  Each link
         // is one aligned 64-bit load (512 B per wavefront instruction); the x-1 / x+1
         // neighbours that sit in the adjacent lane move over by DPP, the two chunk-edge
-        // values come from one broadcast load per shifted link, and both cells' outputs go
-        // out as one 64-bit store per link.
+        // values come from one broadcast load per shifted link (the left edge of a c_x = +1
+        // link, after a fast pair chunk, from lane 63 of the previous chunk in a scalar
+        // register instead), and both cells' outputs go out as one 64-bit store per link.
         const uint32_t npc = (nx + 127) / 128;
         const uint32_t plen = (npc + TL::NSEG - 1) / TL::NSEG;
         const uint32_t pbeg = min(npc, (uint32_t)seg * plen);
         const uint32_t pend = min(npc, pbeg + plen);
+        uint32_t carry[Q] = {};
+        bool carried = false;
 #pragma clang loop unroll(disable)
         for (uint32_t c = pbeg; c < pend; c++) {
             const uint32_t x0 = c * 128;
@@ -670,6 +692,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                         collide_store(x, fl);
                     }
                 }
+                carried = false;
                 continue;
             }
             const uint32_t xa = x0 + 2 * lane;
@@ -677,17 +700,29 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             const uint32_t lb = xl * 4;
             const uint32_t ledge = (x0 >= 1 ? x0 - 1 : 0) * 4;            // cell x0-1 (x0 = 0: the inlet, rewritten)
             const uint32_t redge = (x0 + 128 < nx ? x0 + 128 : x0) * 4;   // cell x0+128
+            uint32_t left[Q];
+            if (carried) {
+                static_for<0, Q>([&](auto I) {
+                    if constexpr (CX[I] > 0) left[I] = carry[I];
+                });
+            } else {
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    if constexpr (CX[i] > 0) left[i] = mem.ld32(srow[i] + ES, ledge);
+                });
+            }
             T fa[Q], fb[Q];
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
                 const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
-                const uint64_t cur = mem.ld64(row, lb);
+                const uint64_t cur = mem.template ld64<TL::NTL>(row, lb);
                 const uint32_t lo = (uint32_t)cur, hi = (uint32_t)(cur >> 32);
                 uint32_t va, vb;
                 if constexpr (CX[i] == 0) {
                     va = lo; vb = hi;
                 } else if constexpr (CX[i] > 0) {                        // cells (x-1, x)
-                    va = wave_shr1(hi, mem.ld32(row, ledge)); vb = lo;
+                    va = wave_shr1(hi, left[i]); vb = lo;
+                    carry[i] = __builtin_amdgcn_readlane(hi, 63);       // cell x0+127
                 } else {                                                 // cells (x+1, x+2)
                     va = hi;
                     vb = wave_shl1(lo, mem.ld32(row, redge));
@@ -719,6 +754,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                     mem.template st64<TL::NT>((size_t)i * NB + selfb, xa * 4, ob[i]);
                 });
             }
+            carried = true;
         }
       }
     } else if constexpr (PF) {
