@@ -345,6 +345,26 @@ def test_pair_path_inlet_chunk(gpu, variant, storage):
         nat.call("sw_set_step_variant", nat.DTYPES[storage], old)
 
 
+@pytest.mark.parametrize("variant", [8, 9, 10])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "f32"])
+def test_pair_edge_carry_across_chunks(gpu, variant, storage):
+    """Pair tiles take the left chunk edge of the c_x = +1 links from the previous fast
+    chunk.  Rows of five 128-cell chunks with the wall in chunk 1: within the wall's height
+    the carry restarts after the special chunk (chunk 2 loads its edge, chunks 3 and 4
+    carry), above it every chunk after the first carries; with rows split over 2 or 4
+    wavefronts each range starts with a load."""
+    from stonewall3d import _native as nat
+    old = nat.lib().sw_get_step_variant(nat.DTYPES[storage])
+    nat.call("sw_set_step_variant", nat.DTYPES[storage], variant)
+    try:
+        flag, uin = small_case(nx=640, ny=5, nz=12, X0=200, lone=False)
+        sim, f, F = run_both(flag, uin, 10, storage, 1, gust=gust_table(make_gust(5), 10))
+        assert not np.where(fluid_mask(flag), sim.populations() != f, False).any()
+        assert np.allclose(sim.force_rows(), F, rtol=1e-12, atol=1e-15 * np.abs(F).max())
+    finally:
+        nat.call("sw_set_step_variant", nat.DTYPES[storage], old)
+
+
 def test_q_frame_bit_identical(gpu):
     """Max-pooled Q-criterion frames (k_qcoarse, lbm3d.py:203-238) equal the oracle's on the
     same macroscopic field."""
