@@ -541,7 +541,8 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         // 128-cell chunks; lane L owns cells xa = x0 + 2L and xa + 1 (nx is even).  Each link
         // is one aligned 32-bit load of two 16-bit values; the value that sits in the
         // adjacent lane comes over by DPP and a 16-bit alignbit rebuilds the shifted pair;
-        // the chunk-edge values come from one broadcast load per shifted link.  (Issuing the
+        // the chunk-edge values come from one broadcast load per shifted link, or for the
+        // left edge of a c_x = +1 link from the previous chunk (carry, below).  (Issuing the
         // next chunk's loads before this chunk's collisions needs 133 VGPRs, 3 wavefronts
         // per SIMD, and measured 13 % slower.)
         const uint32_t npc = (nx + 127) / 128;

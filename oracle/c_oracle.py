@@ -35,8 +35,8 @@ def lib():
         L.oracle_f32_to_f16.restype = ctypes.c_uint16
         L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
         L.oracle_f16_to_f32.restype = f
-        L.oracle_f32_to_bf16.argtypes = [f]
-        L.oracle_f32_to_bf16.restype = ctypes.c_uint16
+        L.oracle_bf16_encode.argtypes = [f]
+        L.oracle_bf16_encode.restype = ctypes.c_uint16
         L.oracle_bf16_to_f32.argtypes = [ctypes.c_uint16]
         L.oracle_bf16_to_f32.restype = f
         L.oracle_qcoarse.argtypes = [p, p, i, i, i, i, i, i, i, i, p]

@@ -110,11 +110,14 @@ float oracle_f16_to_f32(uint16_t h) {
     return u2f(sign | ((ex - 15u + 127u) << 23) | (man << 13));
 }
 
-uint16_t oracle_f32_to_bf16(float f) {
+/* BF16 rounding of the 16-bit storage form (the build's own definition; the reference
+ * stores FP32): dithered rounding.  The float's bits b are truncated to their top 16 after
+ * adding the dither r = (b * 0x9E3779B1 mod 2^32) >> 16 (Knuth's multiplicative hash of b),
+ * so the magnitude rounds up iff low16(b) + r >= 2^16.  Values already in BF16 and +-0 are
+ * unchanged; finite inputs only. */
+uint16_t oracle_bf16_encode(float f) {
     uint32_t u = f2u(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    return (uint16_t)((u + ((u * 0x9E3779B1u) >> 16)) >> 16);
 }
 
 float oracle_bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
@@ -533,7 +536,8 @@ int oracle_qcoarse(const float* U, const uint8_t* flag, int nx, int ny, int nz, 
  * normalised equilibrium of a streamwise velocity U(z), which depends on the direction only
  * through c = c_x,i:
  *     sigma_c = ((1 + 3cU) + (0.5 * 3cU) * 3cU) - 1.5 U^2          (lbm3d.py:35-36 order)
- * as q_i = RN16((phi_i - sigma_c(z)) * scale), scale a power of two, sigma from a table
+ * as q_i = R16((phi_i - sigma_c(z)) * scale) (R16: FP16 round to nearest even, BF16 the
+ * dithered rounding of oracle_bf16_encode), scale a power of two, sigma from a table
  * [nz][4] = (sigma_0, sigma_+1, sigma_-1, 0) (NULL: U = 0, sigma = 1, i.e. (f_i - w_i) / w_i).
  * With U(z) the inlet profile the stored values are small wherever the flow is close to
  * the inlet profile, and so is their rounding error.
@@ -563,7 +567,7 @@ static inline float dec16(const void* f, size_t idx, int st, float s, float inv_
 }
 static inline void enc16(void* f, size_t idx, int st, float v, float s, float scale) {
     float d = (v - s) * scale;
-    ((uint16_t*)f)[idx] = (st == ST_F16) ? oracle_f32_to_f16(d) : oracle_f32_to_bf16(d);
+    ((uint16_t*)f)[idx] = (st == ST_F16) ? oracle_f32_to_f16(d) : oracle_bf16_encode(d);
 }
 /* feq_i / w_i (rho = 1) grouped as ((1 + (0.5 cu) cu) - usq15) + cu, cu = 3 c.u,
  * usq15 = 1.5 * ((ux ux + uy uy) + uz uz) */

@@ -293,17 +293,33 @@ def test_f16_codec_exhaustive(oracle_lib):
 
 
 def test_bf16_codec(oracle_lib):
+    """BF16 storage rounding (dithered): values already in BF16 and +-0 are kept; every other
+    value goes to one of its two BF16 neighbours; and for values a fraction p of an ulp above
+    a BF16 number the share rounded up is p (round to nearest would give 0 or 1, a dither
+    correlated with the discarded bits would miss p), so the mean rounding error is ~0."""
     L = co.lib()
+    enc = lambda v: np.array([L.oracle_bf16_encode(float(t)) for t in v], np.uint16)   # noqa: E731
     rng = np.random.default_rng(1)
+    # kept: BF16 values (random upper halves, zero lower halves) and signed zeros
+    hb = rng.integers(0x0080, 0x7F00, 4000).astype(np.uint32) | (rng.integers(0, 2, 4000).astype(np.uint32) << 15)
+    exact = (hb << 16).view(np.float32)
+    assert np.array_equal(enc(exact), hb.astype(np.uint16))
+    assert L.oracle_bf16_encode(0.0) == 0 and L.oracle_bf16_encode(-0.0) == 0x8000
+    # one of the two neighbours: the truncated upper half or the next one up in magnitude
     x = np.concatenate([rng.standard_normal(20000).astype(np.float32) * s for s in (1e-30, 1e-5, 1, 1e30)])
-    u = x.view(np.uint32).astype(np.uint64)
-    # round to nearest even on the upper 16 bits (the bfloat16 definition)
-    lower = u & 0xFFFF
-    upper = u >> 16
-    up = (lower > 0x8000) | ((lower == 0x8000) & ((upper & 1) == 1))
-    ref = ((upper + up) & 0xFFFF).astype(np.uint16)
-    ours = np.array([L.oracle_f32_to_bf16(float(v)) for v in x], np.uint16)
-    assert np.array_equal(ours, ref)
+    d = enc(x).astype(np.int64) - (x.view(np.uint32) >> 16).astype(np.int64)
+    assert set(np.unique(d)) <= {0, 1}
+    # the share rounded up follows the discarded fraction
+    for p in (0.1, 0.3, 0.5, 0.7, 0.9):
+        n = 20000
+        up16 = rng.integers(0x3000, 0x4F00, n).astype(np.uint32)               # |v| in [2^-31, 2^31)
+        low = np.clip(np.round(p * 65536) + rng.integers(-64, 65, n), 0, 65535).astype(np.uint32)
+        v = ((up16 << 16) | low).view(np.float32)
+        share = (enc(v).astype(np.uint32) != up16).mean()
+        assert abs(share - p) < 0.02, (p, share)
+        err = (enc(v).astype(np.uint32) << 16).view(np.float32).astype(np.float64) - v.astype(np.float64)
+        ulp = ((up16 + 1) << 16).view(np.float32).astype(np.float64) - (up16 << 16).view(np.float32)
+        assert abs((err / ulp).mean()) < 0.02, (p, (err / ulp).mean())
     assert L.oracle_bf16_to_f32(0x3F80) == 1.0
 
 
@@ -378,7 +394,7 @@ def _half_ulp(q, st):
 def test_16bit_step_matches_fp64_restatement(oracle_lib, st, scale, ybc):
     """One 16-bit step (inlet offsets) from a noisy state equals the independent FP64 NumPy
     step of the decoded inputs within the store rounding of each output (half an ulp of the
-    stored value) plus FP32 arithmetic (1.5e-7): every streamed link (offset of the source
+    stored value for FP16, one ulp for the dithered BF16) plus FP32 arithmetic (1.5e-7): every streamed link (offset of the source
     height), the top and side mirrors, the bounce-backs and the inlet decode and collide
     right.  Decoding any link with another height's offset misses by ~1e-4."""
     flag, uin, sh = _inlet_case()
@@ -388,7 +404,9 @@ def test_16bit_step_matches_fp64_restatement(oracle_lib, st, scale, ybc):
     g, F = co.step(f, flag, uin, NU0, 0.16, 10, 0.05, gust=1.1, ybc=ybc, storage=st, scale=scale, shift=sh)
     got = co.decode(g, st, scale, shift=sh).astype(np.float64)
     ref, Fref = nl.step(fin, flag, uin, NU0, 0.16, 10, 0.05, gust=np.float32(1.1), ybc=ybc)
-    bound = nl.W.reshape(19, 1, 1, 1) * _half_ulp(g, st) / scale * 1.01 + 1.5e-7
+    # FP16 rounds to nearest (half an ulp); BF16 rounds dithered (up to one ulp)
+    ulps = 1.0 if st == "f16" else 2.0
+    bound = nl.W.reshape(19, 1, 1, 1) * ulps * _half_ulp(g, st) / scale * 1.01 + 1.5e-7
     err = np.where(fluid, np.abs(got - ref) - bound, -1)
     assert err.max() <= 0, (st, float(err.max()))
     assert np.abs(F - Fref).max() <= 1e-6 * np.abs(Fref).max()
