@@ -27,7 +27,9 @@
  *     [nz][4] float (device; sw_shift_table) or 1 when `shift` is NULL.  The collision runs
  *     on phi in FP32 (moments from opposite-pair sums and differences; velocity, stress
  *     and relaxation time as the reference; outputs rho feq_i / w_i + (keep * 4.5) qq_i
- *     from terms shared across directions).
+ *     from terms shared across directions).  FP16 values are rounded to nearest even;
+ *     BF16 values are rounded dithered: the float's bits b, plus
+ *     (b * 0x9E3779B1 mod 2^32) >> 16, truncated to their top 16.
  *     The same table must be passed to every call on one population state.
  *   - A lattice holds at most 113,025,455 cells.
  */
