@@ -153,7 +153,7 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
         // direction a, (K n_aa + K n_bb - Kt) +- 2 K n_ab for a diagonal one in the ab plane,
         // and -Kt at rest; rho E_i = rho (1 - 1.5 u.u) + (0.5 rho cu) cu +- rho cu.  Per
         // opposite pair: R = rho (1 - 1.5 u.u) + K qq, B = fma(0.5 rho cu, cu, R),
-        // phi_out = fma(+-rho, cu, B).  75 operations for the 19 outputs.
+        // phi_out = fma(+-rho, cu, B).  74 operations for the 19 outputs.
         static_assert(CX[1] == 1 && CY[3] == 1 && CZ[5] == 1, "axis pair leaders");
         static_assert(CX[7] == 1 && CY[7] == 1 && CX[9] == 1 && CY[9] == -1, "xy pair leaders");
         static_assert(CX[11] == 1 && CZ[11] == 1 && CX[13] == 1 && CZ[13] == -1, "xz pair leaders");
