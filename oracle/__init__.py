@@ -19,6 +19,13 @@ Parity pins (what checks each oracle function; see tests/test_oracle_*.py):
                         Cs = 0 -> plain regularized BGK, shear-wave decay rate
   step force            zero at rest; single-link closed form
   step (whole)          C FP64 vs the NumPy restatement; C FP32 vs C FP64
-  macro / accum         closed form from the same moments
-  FP16 / BF16 codecs    numpy's float16 and the bfloat16 definition, exhaustively
+  macro / accum         the NumPy restatement (FP64 and FP32); the noise-free initial
+                        state's closed form (rho = 1, u = (uin, 0, 0)); accumulation twice
+                        = 2 U exactly; 16-bit macro = macro of the decoded populations
+  Q-criterion frames    closed forms (rigid rotation Q = w^2, pure strain pooled to 0,
+                        zero at and next to solids) and the NumPy restatement from the
+                        symmetric / antisymmetric split of the velocity gradient
+  FP16 codec            numpy's float16, exhaustively
+  BF16 dithered codec   its properties (BF16 values kept, one of the two neighbours, share
+                        rounded up = discarded fraction)
 """
