@@ -156,6 +156,33 @@ def test_fp64_beyond_4gib_matches_oracle(gpu):
     assert np.allclose(sim.force_rows(), np.array(forces), rtol=1e-12, atol=1e-16)
 
 
+@pytest.mark.parametrize("dims,wide", [((608, 320, 288), False), ((640, 320, 280), True)])
+def test_fp32_at_the_4gib_offset_limit(gpu, dims, wide):
+    """FP32 on both sides of the 4 GiB reach of 32-bit buffer offsets: 608x320x288 (19 * N *
+    4 B = 4.26 GB, the raw-buffer pair tile with row offsets close to 2^32) and 640x320x280
+    (4.36 GB, the 64-bit-pointer kernel).  Two steps bit-identical to the FP32 oracle."""
+    nx, ny, nz = dims
+    w = workloads.box_wall_flow(n_steps=2, nx=nx, ny=ny, nz=nz, H=40, X0=80, T=12)
+    assert (19 * w.cells * 4 >= 1 << 32) == wide
+    print(f"fp32 {nx}x{ny}x{nz}: {19 * w.cells * 4 / 1e9:.2f} GB per population array", flush=True)
+    sim = Simulation(w.flag, w.uin, dtype="f32", **w.sim_kwargs())
+    sim.init(noise=0.02, seed=6)
+    f = co.init(w.flag, w.uin, "f32", noise=0.02, seed=6)
+    yb = 1 if w.ybc == "freeslip" else 0
+    forces = []
+    for t in range(2):
+        f, F = co.step(f, w.flag, w.uin, w.nu0, w.cs, w.x_sponge, w.nu_sponge, ybc=yb)
+        forces.append(F)
+    sim.step(2)
+    g = sim.populations()
+    fl = (w.flag == 0)[None]
+    differ = 0
+    for i in range(19):                                  # one direction at a time: bounded host memory
+        differ += int(((g[i] != f[i]) & fl[0]).sum())
+    assert differ == 0
+    assert np.allclose(sim.force_rows(), np.array(forces), rtol=1e-12, atol=1e-15 * np.abs(forces).max())
+
+
 def test_macro_and_accumulate_bit_identical(gpu):
     flag, uin = small_case()
     sim, f, _ = run_both(flag, uin, 5, "f32", 0)
