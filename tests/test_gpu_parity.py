@@ -156,6 +156,40 @@ def test_fp64_beyond_4gib_matches_oracle(gpu):
     assert np.allclose(sim.force_rows(), np.array(forces), rtol=1e-12, atol=1e-16)
 
 
+@pytest.mark.parametrize("dims", [(2, 1, 2), (3, 1, 3), (2, 2, 5), (5, 1, 4), (8, 1, 3), (130, 1, 3), (131, 2, 3),
+                                  (132, 2, 3), (257, 3, 4)])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "f64"])
+def test_degenerate_and_ragged_lattices(gpu, dims, storage):
+    """The smallest lattices the ABI accepts (nx = 2, ny = 1, nz = 2: one inlet and one
+    outlet cell above the ground), a single y row (periodic and mirrored onto itself), odd
+    row lengths (the pair tiles' in-kernel fallback), rows of 8 and 132 cells (16-bit
+    quads), a partial last 128-cell chunk and a stone in a one-row lattice: every tile
+    variant the storage form maps to, both y boundaries, 6 steps bit-identical to the
+    oracle (FP64 within 1e-14)."""
+    from stonewall3d import _native as nat
+    nx, ny, nz = dims
+    wall = np.zeros(dims, bool)
+    if nx > 8:
+        wall[nx // 2, 0, 1:nz - 1] = True                 # a stone column in the first y row
+    flag = flag_from_mask(wall)
+    uin = power_law_profile(nz, max(1, nz - 2))
+    old = nat.lib().sw_get_step_variant(nat.DTYPES[storage])
+    try:
+        for variant in ([1] if storage == "f64" else [0, 7, 8, 9, 12]):
+            nat.call("sw_set_step_variant", nat.DTYPES[storage], variant)
+            for ybc in (0, 1):
+                sim, f, F = run_both(flag, uin, 6, storage, ybc, gust=gust_table(make_gust(3), 6))
+                g = sim.populations()
+                fl = fluid_mask(flag)
+                if storage == "f64":
+                    assert np.abs(np.where(fl, g - f, 0)).max() <= 1e-14 * np.abs(f).max()
+                else:
+                    assert not np.where(fl, g != f, False).any(), (dims, storage, variant, ybc)
+                assert np.allclose(sim.force_rows(), F, rtol=1e-12, atol=1e-15 * max(np.abs(F).max(), 1e-300))
+    finally:
+        nat.call("sw_set_step_variant", nat.DTYPES[storage], old)
+
+
 @pytest.mark.parametrize("dims,wide", [((608, 320, 288), False), ((640, 320, 280), True)])
 def test_fp32_at_the_4gib_offset_limit(gpu, dims, wide):
     """FP32 on both sides of the 4 GiB reach of 32-bit buffer offsets: 608x320x288 (19 * N *
