@@ -54,6 +54,46 @@ def test_alone_equals_mix(gpu, dtype, shift, variant):
     b.close()
 
 
+def ragged_workloads():
+    """Lattices of very different shapes in one batch: the smallest accepted (2x1x2), odd
+    rows with a stone, a partial last 128-cell chunk, and a porous wall; step counts differ
+    so the lattices finish in different chunks."""
+    from stonewall3d.inflow import gust_table, make_gust, power_law_profile
+    from stonewall3d.lattice import flag_from_mask
+    out = []
+    for k, (dims, steps) in enumerate((((2, 1, 2), 13), ((131, 2, 3), 21), ((257, 3, 4), 17))):
+        wall = np.zeros(dims, bool)
+        if dims[0] > 8:
+            wall[dims[0] // 2, 0, 1:dims[2] - 1] = True
+        nz = dims[2]
+        out.append(workloads.Workload(f"ragged {dims}", flag_from_mask(wall), power_law_profile(nz, max(1, nz - 2)),
+                                      1, 0.05, 2e-4, x_sponge=dims[0] - 10, ybc="periodic" if k % 2 else "freeslip",
+                                      gust=gust_table(make_gust(k), steps), n_steps=steps))
+    out.append(wl(6, 0.2, 11, 25))
+    return out
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16", "bf16"])
+def test_alone_equals_mix_ragged_default_tiles(gpu, dtype):
+    """The library's default tiles (pairs over two wavefronts per row, chunk edges carried)
+    in a batch of ragged lattices: each result identical to the lattice run alone."""
+    K = 8
+    variant = nat.lib().sw_get_step_variant(nat.DTYPES[dtype])
+    ws = ragged_workloads()
+    seeds = [21, 22, 23, 24]
+    ref = [alone(w, dtype, variant, s) for w, s in zip(ws, seeds)]
+    b = Batch(dtype, max_instances=8, K=K, variant=variant, graph=True)
+    lats = [lattice(w, dtype, s, K) for w, s in zip(ws, seeds)]
+    for lat in lats:
+        b.add(lat)
+    b.run_to_completion()
+    for lat, (p, F) in zip(lats, ref):
+        assert lat.t == lat.n_steps
+        assert np.array_equal(lat.populations(), p)
+        assert np.array_equal(lat.force_rows(), F)
+    b.close()
+
+
 def test_graph_replays_equal_eager(gpu):
     K = 8
     ws = [wl(*c) for c in CASES]
