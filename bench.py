@@ -7,9 +7,10 @@ steps, the FP64 force-row reduction).  Inputs are resident in HBM before timing 
 
 Prints ONE JSON line: MLUPS over all nx*ny*nz cells (as lbm3d.py:384 counts them), the
 step kernel's roofline figure (algorithmic 152 B per FP32 cell update, i.e. 19 loads +
-19 stores of 4 B; 76 B for the 16-bit forms) against 8 TB/s, and a CPU baseline: the
-FP32 oracle (a C restatement of the reference kernel) timed on the host on a bounded
-sample of the same lattice.
+19 stores of 4 B; 76 B for the 16-bit forms) against 8 TB/s, the FP16-storage form timed
+the same way on the same lattice after the FP32 run (`fp16_storage`, with its ratio to the
+FP32 MLUPS), and a CPU baseline: the FP32 oracle (a C restatement of the reference
+kernel) timed on the host on a bounded sample of the same lattice.
 """
 import argparse
 import json
@@ -70,6 +71,32 @@ def cpu_baseline(w, budget_s=15.0, max_steps=40):
                                    sample=f"{steps1} step(s) of the same lattice, one thread, {el1:.1f} s"))
 
 
+def timed_steps(w, dtype, shift, warmup, steps):
+    """Build the lattice in HBM, take `warmup` untimed steps, then time exactly `steps` steps:
+    HIP events around the whole run and around every launch, on the launch stream.
+    Returns (simulation, total ms, mean ms per step launch)."""
+    import numpy as np
+    import torch
+    from stonewall3d.lbm3d import Simulation
+    sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, **w.sim_kwargs())
+    sim.init(noise=0.02, seed=0)
+    sim.step(warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(steps):
+        ev[k][0].record(stream)
+        sim.step(1)
+        ev[k][1].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    return sim, t_start.elapsed_time(t_end), float(np.mean([s.elapsed_time(e) for s, e in ev]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,7 +110,9 @@ def main():
                          "lbm3d.py's own default case (20H x 4H x (8H+1), its stone wall, seed 7, periodic y)")
     ap.add_argument("--H", type=int, default=40)
     ap.add_argument("--porosity", type=float, default=0.2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="time the steps of --dtype only: no CPU baseline and no FP16-storage comparison leg "
+                         "(the form the rocprofv3 runs use)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy one FP32 population array (19 x 640 x 320 x 160 x 4 B) 5 times: the known-size "
@@ -106,7 +135,7 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}))
         return
     from stonewall3d import workloads
-    from stonewall3d.lbm3d import Simulation, bytes_per_update
+    from stonewall3d.lbm3d import bytes_per_update
 
     n_total = a.warmup + a.steps
     if a.workload == "reference":
@@ -119,26 +148,7 @@ def main():
                 f"free-slip top/sides")
     nx, ny, nz = w.shape
     N = nx * ny * nz
-    sim = Simulation(w.flag, w.uin, dtype=a.dtype, shift=a.shift, **w.sim_kwargs())
-    sim.init(noise=0.02, seed=0)
-    sim.step(a.warmup)
-    torch.cuda.synchronize()
-
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_start.record(stream)
-    for k in range(a.steps):
-        ev[k][0].record(stream)
-        sim.step(1)
-        ev[k][1].record(stream)
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    total_ms = t_start.elapsed_time(t_end)
-    launch_ms = [s.elapsed_time(e) for s, e in ev]
-    kern_ms = float(np.mean(launch_ms))
+    sim, total_ms, kern_ms = timed_steps(w, a.dtype, a.shift, a.warmup, a.steps)
     F = sim.force_rows()
     if not np.isfinite(F).all():
         raise SystemExit("force rows are not finite")
@@ -175,6 +185,21 @@ def main():
     }
     del sim
     torch.cuda.empty_cache()
+    if not a.no_cpu_baseline and a.dtype == "f32":
+        # the metric's FP16-storage side, on the same lattice and step count (outside the timed FP32 region)
+        s16, t16, k16 = timed_steps(w, "f16", a.shift, a.warmup, a.steps)
+        if not np.isfinite(s16.force_rows()).all():
+            raise SystemExit("FP16 force rows are not finite")
+        m16 = a.steps * N / (t16 * 1e-3) / 1e6
+        a16 = N * bytes_per_update("f16") / (k16 * 1e-3) / 1e9
+        line["fp16_storage"] = {"value": round(m16, 1), "unit": "MLUPS", "ratio_vs_fp32": round(m16 / mlups, 3),
+                                "ms_per_step": round(t16 / a.steps, 4), "kernel_avg_ms": round(k16, 4),
+                                "bytes_per_update": bytes_per_update("f16"), "achieved_gbs": round(a16, 1),
+                                "roofline_frac": round(a16 / HBM_PEAK_GBS, 4),
+                                "traffic": traffic_from_profile("f16") if headline else None,
+                                "storage_offset": s16.shift_kind}
+        del s16
+        torch.cuda.empty_cache()
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
     print(json.dumps(line), flush=True)
