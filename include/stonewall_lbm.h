@@ -123,7 +123,9 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
  *   11 9 with cached stores
  *   12 quads (16-bit storage): a lane owns four adjacent cells, one 64-bit access per
  *      link; NT stores                 13  12, rows over 2 wavefronts
- * FP64 storage maps 8..13 to 1, FP32 maps 12..13 to 9.  Arrays of 4 GiB or more use 64-bit addressing with
+ * FP64 storage maps 8..13 to 1, FP32 maps 12..13 to 9.  Variant 9 runs as 8 on short rows
+ * (16-bit storage: nx < 512; FP32: nx <= 128), where splitting a row over two wavefronts
+ * leaves one idle or unbalanced.  Arrays of 4 GiB or more use 64-bit addressing with
  * variant 1's tile whatever the setting. */
 int sw_step_variants(void);
 int sw_set_step_variant(int dtype, int variant);

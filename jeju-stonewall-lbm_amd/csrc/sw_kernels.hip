@@ -316,6 +316,11 @@ static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
     // 64-bit addressing (arrays of 4 GiB and more): plain sweep with nontemporal stores
     if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV1, false>(a, s); return; }
     int v = g_step_variant[dtype];
+    // Variant 9 splits each row's 128-cell pair chunks over two wavefronts; with only a few
+    // chunks per row one of them idles or gets the smaller share, and one wavefront per row
+    // (variant 8) is faster: measured for 16-bit rows below 512 cells and FP32 rows of one
+    // chunk (lattices alone, 20 % void, H = 8 ... 40).
+    if (v == 9 && (sizeof(S) == 2 ? a.nx < 512 : a.nx <= 128)) v = 8;
     if (v >= 8 && sizeof(S) == 8) v = 1;    // pairs: 16/32-bit storage (odd nx falls back in-kernel)
     if ((v == 12 || v == 13) && sizeof(S) == 4) v = 9;   // quads: 16-bit storage only
     switch (v) {
