@@ -123,8 +123,10 @@ class Batch:
     K        : steps per chunk (one graph replay); every lattice's ring must be >= K.
     blocks   : persistent workgroups per step launch (default: 8 per compute unit, enough
                for the step kernels' 6-7 resident wavefronts per SIMD).
-    variant  : tile variant (sw_set_step_variant numbering); default: the library's
-               current choice for this dtype.
+    variant  : tile variant (sw_set_step_variant numbering); default: 8 (pairs, one wavefront
+               per row) for the 16-bit forms, the batch kernel's fastest at every measured
+               lattice size (rows of 128-640 cells), else the library's current choice for
+               this dtype.
     graph    : capture the chunk as a hipGraph (else launch it eagerly).
     """
 
@@ -134,7 +136,9 @@ class Batch:
         self.max_instances, self.K = int(max_instances), int(K)
         self.device = torch.device(device)
         self.blocks = int(blocks) if blocks else default_blocks(self.device)
-        self.variant = lib.sw_get_step_variant(self.dtype) if variant is None else int(variant)
+        if variant is None:
+            variant = 8 if self.dtype in (nat.SW_F16, nat.SW_BF16) else lib.sw_get_step_variant(self.dtype)
+        self.variant = int(variant)
         self.table = torch.zeros(lib.sw_batch_table_bytes(self.max_instances), dtype=torch.uint8, device=self.device)
         self.members = []
         self.big = False
