@@ -319,11 +319,12 @@ def test_headline_lattice_first_steps_bit_identical(gpu):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("dtype", ["f32", "f16"])
+@pytest.mark.parametrize("dtype", ["f32", "f16", "bf16", "f64"])
 def test_resume_is_bitwise_equal_to_uninterrupted(gpu, tmp_path, dtype):
     """Run 30 steps straight vs 12 steps, checkpoint, a fresh Simulation resumes and runs
     18 more: populations, force rows and the accumulator agree bit for bit (the
-    reference's claim, stonewall3d/README.md:44); FP16 with the inlet offsets too."""
+    reference's claim, stonewall3d/README.md:44); in every storage form (the 16-bit ones
+    with the inlet offsets)."""
     from stonewall3d import checkpoint
     flag, uin = small_case()
     kw = dict(nu0=2e-4, x_sponge=38, ybc="periodic", n_steps=30, ring=5, gust=gust_table(make_gust(3), 30),
