@@ -37,7 +37,8 @@ def lattice(w, dtype, seed, K, shift=None):
 
 
 @pytest.mark.parametrize("variant", [VARIANT, 8])
-@pytest.mark.parametrize("dtype,shift", [("f32", None), ("f16", "inlet"), ("f16", "weights"), ("bf16", "inlet")])
+@pytest.mark.parametrize("dtype,shift", [("f32", None), ("f16", "inlet"), ("f16", "weights"), ("bf16", "inlet"),
+                                         ("bf16", "weights")])
 def test_alone_equals_mix(gpu, dtype, shift, variant):
     K = 8
     ws = [wl(*c) for c in CASES]
