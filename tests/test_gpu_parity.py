@@ -319,6 +319,35 @@ def test_headline_lattice_first_steps_bit_identical(gpu):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_reference_default_case_full_size(gpu, storage):
+    """The reference's own default lattice (lbm3d.py at H = 40: 800x160x321, its stone
+    generator at 20 %, seed 7, periodic y, no gust; odd nz): 2 steps bit-identical to the
+    oracle at full size."""
+    w = workloads.reference(H=40, porosity=0.2, seed=7, n_steps=2)
+    assert w.shape == (800, 160, 321)
+    co.set_threads(16)
+    sc = SCALE[storage]
+    sim = Simulation(w.flag, w.uin, dtype=storage, scale=sc, **w.sim_kwargs())
+    sim.init(noise=0.02, seed=7)
+    sim.step(2)
+    sh = oracle_shift(w.uin, storage, None)
+    f = co.init(w.flag, w.uin, storage, noise=0.02, seed=7, scale=sc, shift=sh)
+    yb = 1 if w.ybc == "freeslip" else 0
+    F = []
+    for t in range(2):
+        f, Ft = co.step(f, w.flag, w.uin, w.nu0, w.cs, w.x_sponge, w.nu_sponge, ybc=yb, storage=storage, scale=sc,
+                        shift=sh)
+        F.append(Ft)
+    g = sim.populations()
+    fl = (w.flag == 0)
+    differ = sum(int(((g[i] != f[i]) & fl).sum()) for i in range(19))
+    assert differ == 0
+    assert np.allclose(sim.force_rows(), np.array(F), rtol=1e-11, atol=1e-14 * np.abs(F).max())
+    del sim
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("dtype", ["f32", "f16", "bf16", "f64"])
 def test_resume_is_bitwise_equal_to_uninterrupted(gpu, tmp_path, dtype):
     """Run 30 steps straight vs 12 steps, checkpoint, a fresh Simulation resumes and runs
