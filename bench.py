@@ -2,15 +2,27 @@
 
 Workload (default): the H=40 porous stone wall at 20 % void fraction, 640 x 320 x 160
 cells (32.8 M), gusting inlet, nu = 2e-4, Cs = 0.16, free-slip top and sides, FP32
-populations.  A step is one fused kernel launch over the whole lattice (plus, every 64
-steps, the FP64 force-row reduction).  Inputs are resident in HBM before timing starts.
+populations.  A step is one fused kernel launch over the whole lattice, plus every 10
+steps the FP64 force-row reduction (inside the timed region).  Inputs are resident in
+HBM before timing starts.
 
-Prints ONE JSON line: MLUPS over all nx*ny*nz cells (as lbm3d.py:384 counts them), the
-step kernel's roofline figure (algorithmic 152 B per FP32 cell update, i.e. 19 loads +
-19 stores of 4 B; 76 B for the 16-bit forms) against 8 TB/s, the FP16-storage form timed
-the same way on the same lattice after the FP32 run (`fp16_storage`, with its ratio to the
-FP32 MLUPS), and a CPU baseline: the FP32 oracle (a C restatement of the reference
-kernel) timed on the host on a bounded sample of the same lattice.
+Prints ONE JSON line:
+  value / ms_per_step   MLUPS over all nx*ny*nz cells (as lbm3d.py:384 counts them) from
+                        the HIP-event time of the whole timed region
+  roofline              achieved = algorithmic bytes (152 B per FP32 cell update: 19 loads
+                        + 19 stores of 4 B; 76 B for the 16-bit forms) / ms_per_step, against
+                        8 TB/s; kernel_avg_ms is the step kernel's mean launch duration
+                        (events around each launch); traffic is the HBM bytes per launch from
+                        the rocprofv3 PMC summary in profiles/ (FETCH_SIZE and WRITE_SIZE,
+                        scaled by the same counters on a copy of known size)
+  copy_ceiling          the same process's streaming copy of one population array (the
+                        read + write pattern of a step without arithmetic), and the step's
+                        fraction of it
+  fp16_storage          the FP16-storage form timed the same way on the same lattice after
+                        the FP32 run, with its ratio to the FP32 MLUPS
+  cpu_baseline          the FP32 oracle (a C restatement of the reference kernel) on the
+                        host: a bounded sample of the same lattice on all cores (value) and
+                        on one thread, plus the 160 x 80 x 40 box-wall case
 """
 import argparse
 import json
@@ -27,12 +39,14 @@ HBM_PEAK_GBS = 8000.0
 
 
 def traffic_from_profile(dtype):
-    """HBM bytes per step launch measured by rocprofv3 PMC counters, if a summary exists."""
-    p = os.path.join(ROOT, "profiles", f"k_step_porous_wall_640x320x160_{dtype}_traffic.json")
+    """HBM bytes per step launch and the summary they come from (rocprofv3 PMC counters
+    FETCH_SIZE / WRITE_SIZE of k_step on the 640x320x160 lattice), if the summary exists."""
+    name = f"k_step_porous_wall_640x320x160_{dtype}_traffic.json"
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
-    return None
+            return json.load(f).get("bytes_per_launch"), f"profiles/{name} (rocprofv3 PMC)"
+    return None, None
 
 
 def _oracle_rate(co, w, threads, budget_s, max_steps):
@@ -55,30 +69,40 @@ def _oracle_rate(co, w, threads, budget_s, max_steps):
     return steps * nx * ny * nz / el / 1e6, steps, el
 
 
-def cpu_baseline(w, budget_s=15.0, max_steps=40):
-    """FP32 oracle (a C restatement of the reference kernel) on the host, on the same
-    lattice: as many steps as fit in ~budget_s with one OpenMP thread per core (at most 16),
-    plus a one-thread sample of ~budget_s / 2."""
-    from oracle import c_oracle as co
-    threads = min(16, os.cpu_count() or 1)
+def _cpu_entry(co, w, threads, budget_s, max_steps, what):
     v, steps, el = _oracle_rate(co, w, threads, budget_s, max_steps)
-    v1, steps1, el1 = _oracle_rate(co, w, 1, budget_s / 2, 2)
     nx, ny, nz = w.flag.shape
-    return dict(value=v, unit="MLUPS", cores=threads, kind="port",
-                sample=f"{steps} steps of the {nx}x{ny}x{nz} lattice, FP32 C oracle, {threads} OpenMP threads, "
-                       f"{el:.1f} s",
-                single_thread=dict(value=v1, unit="MLUPS", cores=1,
-                                   sample=f"{steps1} step(s) of the same lattice, one thread, {el1:.1f} s"))
+    return dict(value=round(v, 2), unit="MLUPS", gbs=round(v * 1e6 * 152 / 1e9, 2), cores=threads,
+                sample=f"{steps} steps of {what} ({nx}x{ny}x{nz}), FP32 C oracle, {threads} thread(s), {el:.1f} s")
 
 
-def timed_steps(w, dtype, shift, warmup, steps):
+def cpu_baseline(w, budget_s=10.0):
+    """The FP32 oracle (a C restatement of the reference kernel) on the host cores, on
+    bounded samples: the benchmarked lattice on all cores (at most 16 OpenMP threads; the
+    headline value) and on one thread, and the CPU-sized box-wall case (160x80x40: up to its
+    full 2000 steps on all cores, a sample on one thread).  GB/s counts the same 152
+    algorithmic bytes per update as the GPU figure."""
+    from oracle import c_oracle as co
+    from stonewall3d import workloads
+    threads = min(16, os.cpu_count() or 1)
+    head = _cpu_entry(co, w, threads, budget_s, 40, "the benchmarked lattice")
+    one = _cpu_entry(co, w, 1, budget_s / 2, 2, "the benchmarked lattice")
+    c0 = workloads.box_wall_flow(n_steps=2000)
+    c0_all = _cpu_entry(co, c0, threads, budget_s, 2000, "the box-wall case")
+    c0_one = _cpu_entry(co, c0, 1, budget_s / 2, 2000, "the box-wall case")
+    return dict(head, kind="port", single_thread=one,
+                box_wall_160x80x40=dict(all_cores=c0_all, single_thread=c0_one))
+
+
+def timed_steps(w, dtype, shift, warmup, steps, ring=10):
     """Build the lattice in HBM, take `warmup` untimed steps, then time exactly `steps` steps:
-    HIP events around the whole run and around every launch, on the launch stream.
+    HIP events around the whole run and around every launch, on the launch stream (the
+    force-row reduction runs every `ring` steps inside the timed region).
     Returns (simulation, total ms, mean ms per step launch)."""
     import numpy as np
     import torch
     from stonewall3d.lbm3d import Simulation
-    sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, **w.sim_kwargs())
+    sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, ring=ring, **w.sim_kwargs())
     sim.init(noise=0.02, seed=0)
     sim.step(warmup)
     torch.cuda.synchronize()
@@ -97,6 +121,28 @@ def timed_steps(w, dtype, shift, warmup, steps):
     return sim, t_start.elapsed_time(t_end), float(np.mean([s.elapsed_time(e) for s, e in ev]))
 
 
+def copy_ceiling(nbytes, reps=10):
+    """(GB/s of bytes read + bytes written, ms per copy) of the library's streaming copy of
+    nbytes, timed with HIP events over `reps` launches after two untimed ones."""
+    import torch
+    from stonewall3d import _native as nat
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    s = torch.cuda.current_stream()
+    for _ in range(2):
+        nat.call("sw_stream_copy", src.data_ptr(), dst.data_ptr(), nbytes, s.cuda_stream)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        nat.call("sw_stream_copy", src.data_ptr(), dst.data_ptr(), nbytes, s.cuda_stream)
+    b.record(s)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9, ms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,19 +150,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--shift", default="inlet", choices=["inlet", "weights"],
-                    help="16-bit forms: store f_i - feq_i(1, uin[z]) (inlet) or f_i - w_i (weights)")
+                    help="16-bit forms: offsets from the inlet equilibrium per height (inlet) or the weights")
     ap.add_argument("--workload", default="porous_wall", choices=["porous_wall", "reference"],
                     help="porous_wall: 16H x 8H x 4H rounded-box stone wall with a gusting inlet; reference: "
                          "lbm3d.py's own default case (20H x 4H x (8H+1), its stone wall, seed 7, periodic y)")
     ap.add_argument("--H", type=int, default=40)
     ap.add_argument("--porosity", type=float, default=0.2)
     ap.add_argument("--no-cpu-baseline", action="store_true",
-                    help="time the steps of --dtype only: no CPU baseline and no FP16-storage comparison leg "
-                         "(the form the rocprofv3 runs use)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+                    help="time the steps of --dtype only: no CPU baseline, no copy ceiling and no FP16-storage "
+                         "comparison leg (the form the rocprofv3 runs use)")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only copy one FP32 population array (19 x 640 x 320 x 160 x 4 B) 5 times: the known-size "
-                         "copy against which rocprofv3's FETCH_SIZE / WRITE_SIZE counters are scaled")
+                    help="only the streaming copy of one FP32 population array (19 x 640 x 320 x 160 x 4 B), 12 "
+                         "times: the known-size copy against which rocprofv3's FETCH_SIZE / WRITE_SIZE are scaled")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the stone-wall step runs on a single MI355X here "
@@ -126,13 +172,10 @@ def main():
     import numpy as np
     import torch
     if a.calibrate_copy:
-        n = 19 * 640 * 320 * 160
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}))
+        n = 19 * 640 * 320 * 160 * 4
+        gbs, ms = copy_ceiling(n)
+        print(json.dumps({"calibration": "sw_stream_copy", "bytes_read": n, "bytes_written": n, "launches": 12,
+                          "gbs": round(gbs, 1), "ms": round(ms, 4)}))
         return
     from stonewall3d import workloads
     from stonewall3d.lbm3d import bytes_per_update
@@ -153,12 +196,12 @@ def main():
     if not np.isfinite(F).all():
         raise SystemExit("force rows are not finite")
     fluid = int((w.flag == 0).sum())
-    mlups = a.steps * N / (total_ms * 1e-3) / 1e6
+    ms_step = total_ms / a.steps
+    mlups = N / (ms_step * 1e-3) / 1e6
     bpu = bytes_per_update(a.dtype)
-    achieved = N * bpu / (kern_ms * 1e-3) / 1e9
+    achieved = N * bpu / (ms_step * 1e-3) / 1e9
     headline = a.workload == "porous_wall" and a.H == 40 and a.porosity == 0.2
-    traffic = traffic_from_profile(a.dtype) if headline else None
-    cd = sim.coefficients(w.UH, w.H)["Cd"]
+    traffic, traffic_src = traffic_from_profile(a.dtype) if headline else (None, None)
 
     line = {
         "metric": METRIC,
@@ -167,7 +210,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(total_ms / a.steps, 4),
+        "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -178,26 +221,34 @@ def main():
                    "cells": N, "fluid_cells": fluid, "bytes_per_update": bpu,
                    "storage_offset": sim.shift_kind if a.dtype in ("f16", "bf16") else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_step", "kernel_avg_ms": round(kern_ms, 4)},
-        "fluid_mlups": round(a.steps * fluid / (total_ms * 1e-3) / 1e6, 1),
-        "mean_cd": float(cd[a.warmup:].mean()),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "k_step", "kernel_avg_ms": round(kern_ms, 4),
+                     "kernel_frac": round(N * bpu / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "fluid_mlups": round(fluid / (ms_step * 1e-3) / 1e6, 1),
     }
     del sim
     torch.cuda.empty_cache()
+    if not a.no_cpu_baseline:
+        pop_bytes = 19 * N * 4
+        gbs, cms = copy_ceiling(pop_bytes)
+        line["copy_ceiling"] = {"gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+                                "step_frac_of_copy": round(achieved / gbs, 4), "ms": round(cms, 4),
+                                "what": f"sw_stream_copy of one FP32 population array ({pop_bytes / 1e9:.2f} GB), "
+                                        "bytes read + written per second"}
     if not a.no_cpu_baseline and a.dtype == "f32":
         # the metric's FP16-storage side, on the same lattice and step count (outside the timed FP32 region)
         s16, t16, k16 = timed_steps(w, "f16", a.shift, a.warmup, a.steps)
         if not np.isfinite(s16.force_rows()).all():
             raise SystemExit("FP16 force rows are not finite")
-        m16 = a.steps * N / (t16 * 1e-3) / 1e6
-        a16 = N * bytes_per_update("f16") / (k16 * 1e-3) / 1e9
+        ms16 = t16 / a.steps
+        m16 = N / (ms16 * 1e-3) / 1e6
+        a16 = N * bytes_per_update("f16") / (ms16 * 1e-3) / 1e9
+        tr16, src16 = traffic_from_profile("f16") if headline else (None, None)
         line["fp16_storage"] = {"value": round(m16, 1), "unit": "MLUPS", "ratio_vs_fp32": round(m16 / mlups, 3),
-                                "ms_per_step": round(t16 / a.steps, 4), "kernel_avg_ms": round(k16, 4),
+                                "ms_per_step": round(ms16, 4), "kernel_avg_ms": round(k16, 4),
                                 "bytes_per_update": bytes_per_update("f16"), "achieved_gbs": round(a16, 1),
                                 "roofline_frac": round(a16 / HBM_PEAK_GBS, 4),
-                                "traffic": traffic_from_profile("f16") if headline else None,
-                                "storage_offset": s16.shift_kind}
+                                "traffic": tr16, "traffic_source": src16, "storage_offset": s16.shift_kind}
         del s16
         torch.cuda.empty_cache()
     if not a.no_cpu_baseline:

@@ -167,6 +167,12 @@ int sw_qcoarse(const float* U, const uint32_t* links, int nx, int ny, int nz, in
  * 2^32 hashed samples (|a| in [2^-40, 4), b in [2^-4, 4)).  Both must be 0. */
 int sw_selftest_division(unsigned long long* counts, void* stream);
 
+/* Streaming copy dst = src of `bytes` (a multiple of 16; both 16-byte aligned): 16-byte
+ * loads and nontemporal 16-byte stores, eight 256-thread workgroups per compute unit.  No
+ * reference counterpart: it measures the HBM read + write ceiling the step is compared
+ * against (bench.py), and it is the known-size kernel that scales the PMC byte counters. */
+int sw_stream_copy(const void* src, void* dst, long long bytes, void* stream);
+
 /* ---------------------------------------------------------------- batched form
  * No reference counterpart: the reference runs one lattice per process (run_all.sh).
  * Many independent lattices of one dtype advance together: one persistent launch per

@@ -292,6 +292,18 @@ __global__ __launch_bounds__(256) void k_selftest_division(unsigned long long* c
     if (bad_div) atomicAdd(&counts[1], bad_div);
 }
 
+// ------------------------------------------------------------------ streaming copy (bandwidth ceiling)
+// dst = src over n16 16-byte words: one 16-byte load and one nontemporal 16-byte store per
+// lane and iteration, grid-stride.  The same read + write stream a step moves, with no
+// arithmetic and no neighbour access: the HBM ceiling the step is compared against.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                     size_t n16) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride)
+        __builtin_nontemporal_store(src[k], &dst[k]);
+}
+
 }  // namespace sw
 
 using namespace sw;
@@ -330,6 +342,48 @@ static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
         default: launch_step_v<S, TileV1, true>(a, s); break;
     }
 }
+
+namespace sw {
+
+int step_entry(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
+               int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
+               int ybc, float scale, const float* shift, double* partials, int ring, hipStream_t s) {
+    SW_CHECK_ARG(f_in && f_out && links && uin, "f_in, f_out, links and uin must be non-null");
+    SW_CHECK_ARG(f_in != f_out, "f_in and f_out must be different buffers (pull scheme)");
+    SW_CHECK_RC(check_dims(nx, ny, nz));
+    SW_CHECK_RC(check_dtype(dtype, scale));
+    SW_CHECK_ARG(ybc == Y_PERIODIC || ybc == Y_FREESLIP, "ybc must be 0 (periodic) or 1 (free-slip)");
+    SW_CHECK_ARG(!partials || ring > 0, "ring must be positive when partials are given");
+    SW_CHECK_ARG(t >= 0, "t must be non-negative");
+    StepArgs a{};
+    a.f_in = f_in; a.f_out = f_out; a.links = links; a.uin = uin; a.gust = gust;
+    a.partials = partials; a.t_dev = t_dev; a.t = t; a.ring = ring > 0 ? ring : 1;
+    a.shift = (dtype == DT_F16 || dtype == DT_BF16) ? shift : nullptr;
+    a.nx = nx; a.ny = ny; a.nz = nz; a.nyb = (ny + ROWS - 1) / ROWS;
+    a.x_sponge = x_sponge; a.ybc = ybc; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
+    a.scale = scale; a.inv_scale = 1.0f / scale;
+    switch (dtype) {
+        case DT_F32: launch_step<float>(a, dtype, s); break;
+        case DT_F64: launch_step<double>(a, dtype, s); break;
+        case DT_F16: launch_step<_Float16>(a, dtype, s); break;
+        case DT_BF16: launch_step<bf16_t>(a, dtype, s); break;
+    }
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int force_reduce_entry(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int ny, int nz,
+                       double* force, hipStream_t s) {
+    SW_CHECK_ARG(partials && force, "partials and force must be non-null");
+    SW_CHECK_ARG(ring > 0 && nsteps > 0 && nsteps <= ring, "need 0 < nsteps <= ring");
+    SW_CHECK_ARG(ny > 0 && nz > 0 && t0 >= 0, "bad ny/nz/t0");
+    const int nyb = (ny + ROWS - 1) / ROWS;
+    hipLaunchKernelGGL(k_force_reduce, dim3(nz, nsteps), dim3(64), 0, s, partials, ring, t0, t_dev, nz, nyb, force);
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+}  // namespace sw
 
 extern "C" {
 
@@ -396,29 +450,8 @@ int sw_init(void* f, int dtype, const uint32_t* links, const float* uin, int nx,
 int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
             int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
             int ybc, float scale, const float* shift, double* partials, int ring, void* stream) {
-    SW_CHECK_ARG(f_in && f_out && links && uin, "f_in, f_out, links and uin must be non-null");
-    SW_CHECK_ARG(f_in != f_out, "f_in and f_out must be different buffers (pull scheme)");
-    SW_CHECK_RC(check_dims(nx, ny, nz));
-    SW_CHECK_RC(check_dtype(dtype, scale));
-    SW_CHECK_ARG(ybc == Y_PERIODIC || ybc == Y_FREESLIP, "ybc must be 0 (periodic) or 1 (free-slip)");
-    SW_CHECK_ARG(!partials || ring > 0, "ring must be positive when partials are given");
-    SW_CHECK_ARG(t >= 0, "t must be non-negative");
-    StepArgs a{};
-    a.f_in = f_in; a.f_out = f_out; a.links = links; a.uin = uin; a.gust = gust;
-    a.partials = partials; a.t_dev = t_dev; a.t = t; a.ring = ring > 0 ? ring : 1;
-    a.shift = (dtype == DT_F16 || dtype == DT_BF16) ? shift : nullptr;
-    a.nx = nx; a.ny = ny; a.nz = nz; a.nyb = (ny + ROWS - 1) / ROWS;
-    a.x_sponge = x_sponge; a.ybc = ybc; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
-    a.scale = scale; a.inv_scale = 1.0f / scale;
-    hipStream_t s = (hipStream_t)stream;
-    switch (dtype) {
-        case DT_F32: launch_step<float>(a, dtype, s); break;
-        case DT_F64: launch_step<double>(a, dtype, s); break;
-        case DT_F16: launch_step<_Float16>(a, dtype, s); break;
-        case DT_BF16: launch_step<bf16_t>(a, dtype, s); break;
-    }
-    SW_HIP(hipGetLastError());
-    return SW_OK;
+    return step_entry(f_in, f_out, dtype, links, uin, gust, t, t_dev, nx, ny, nz, nu0, cs, x_sponge, nu_sponge, ybc,
+                      scale, shift, partials, ring, (hipStream_t)stream);
 }
 
 int sw_step_variants(void) { return NUM_TILE_VARIANTS; }
@@ -434,14 +467,7 @@ int sw_get_step_variant(int dtype) { return (dtype >= 0 && dtype <= 3) ? g_step_
 
 int sw_force_reduce(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int ny, int nz,
                     double* force, void* stream) {
-    SW_CHECK_ARG(partials && force, "partials and force must be non-null");
-    SW_CHECK_ARG(ring > 0 && nsteps > 0 && nsteps <= ring, "need 0 < nsteps <= ring");
-    SW_CHECK_ARG(ny > 0 && nz > 0 && t0 >= 0, "bad ny/nz/t0");
-    const int nyb = (ny + ROWS - 1) / ROWS;
-    hipLaunchKernelGGL(k_force_reduce, dim3(nz, nsteps), dim3(64), 0, (hipStream_t)stream, partials, ring, t0, t_dev,
-                       nz, nyb, force);
-    SW_HIP(hipGetLastError());
-    return SW_OK;
+    return force_reduce_entry(partials, ring, t0, t_dev, nsteps, ny, nz, force, (hipStream_t)stream);
 }
 
 int sw_macro(const void* f, int dtype, const uint32_t* links, int nx, int ny, int nz, float scale,
@@ -491,6 +517,19 @@ int sw_selftest_division(unsigned long long* counts, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, counts);
     hipLaunchKernelGGL(k_selftest_division, dim3(16384), dim3(256), 0, s, counts);
+    SW_HIP(hipGetLastError());
+    return SW_OK;
+}
+
+int sw_stream_copy(const void* src, void* dst, long long bytes, void* stream) {
+    SW_CHECK_ARG(src && dst && src != dst, "src and dst must be distinct non-null buffers");
+    SW_CHECK_ARG(bytes > 0 && bytes % 16 == 0, "bytes must be a positive multiple of 16");
+    SW_CHECK_ARG(((uintptr_t)src | (uintptr_t)dst) % 16 == 0, "src and dst must be 16-byte aligned");
+    int dev = 0, cus = 0;
+    SW_HIP(hipGetDevice(&dev));
+    SW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)(cus * 8)), dim3(256), 0, (hipStream_t)stream,
+                       (const u32x4*)src, (u32x4*)dst, (size_t)bytes / 16);
     SW_HIP(hipGetLastError());
     return SW_OK;
 }

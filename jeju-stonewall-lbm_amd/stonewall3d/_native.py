@@ -38,6 +38,7 @@ SIGNATURES = {
     "sw_shift_table": (_c_int, [_p, _c_int, _c_int, _p, _p]),
     "sw_qcoarse": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "sw_selftest_division": (_c_int, [_p, _p]),
+    "sw_stream_copy": (_c_int, [_p, _p, _c_ll, _p]),
     "sw_batch_table_bytes": (_c_ll, [_c_int]),
     "sw_batch_upload": (_c_int, [_p, _c_int, _c_int, _p, _p]),
     "sw_batch_download": (_c_int, [_p, _p, _c_int, _p]),

@@ -53,7 +53,11 @@ class Lattice:
         self.shift = storage_shift(self.uin, self.dtype, self.shift_kind)
         self.gust = None
         if gust is not None:
-            self.gust = torch.from_numpy(np.ascontiguousarray(gust[:self.n_steps], np.float32)).to(dev)
+            g = np.ascontiguousarray(gust, np.float32)
+            if g.ndim != 1 or g.shape[0] < self.n_steps:
+                raise ValueError(f"gust table has {g.shape[0] if g.ndim else 0} entries; the lattice's "
+                                 f"{self.n_steps} steps read one per step")
+            self.gust = torch.from_numpy(g[:self.n_steps].copy()).to(dev)
         self.partials = torch.zeros(self.ring * lib.sw_partials_per_step(self.ny, self.nz), dtype=torch.float64,
                                     device=dev)
         self.force = torch.zeros((self.n_steps, self.nz, 2), dtype=torch.float64, device=dev)

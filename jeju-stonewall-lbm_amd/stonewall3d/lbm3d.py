@@ -331,15 +331,45 @@ __all__ = ["Simulation", "bytes_per_update", "FLUID", "WALL", "GROUND"]
 
 
 # ---------------------------------------------------------------------------------- CLI
+REFERENCE_SIG_KEYS = ("grid", "H", "porosity", "seed", "Re", "U0", "Cs", "n_spin", "n_total", "snap_every", "snap3d")
+
+
+def run_signature(a, grid, n_spin, n_total, snap_every, ds3):
+    """Checkpoint signature of a CLI run (lbm3d.py:324-325).  With the reference's storage
+    (FP32), side boundary (periodic) and steady inlet it holds exactly the reference's eleven
+    keys, so the reference and this package accept each other's checkpoints; each setting
+    that departs from those adds its own key, because such populations are not
+    interchangeable with the reference's."""
+    sig = dict(grid=list(grid), H=a.H, porosity=a.porosity, seed=a.seed, Re=a.Re, U0=a.U0, Cs=a.Cs, n_spin=n_spin,
+               n_total=n_total, snap_every=snap_every, snap3d=[a.snap3d_every, ds3])
+    if a.dtype != "f32":
+        sig["dtype"] = a.dtype
+        if a.dtype in ("f16", "bf16"):
+            sig["shift"] = a.shift
+    if a.ybc != "periodic":
+        sig["ybc"] = a.ybc
+    if a.gust_seed is not None:
+        sig["gust_seed"] = a.gust_seed
+    return sig
+
+
+def divergence_guard(U, case, t):
+    """The run's NaN check (lbm3d.py:386-391): stops the run with SystemExit when the
+    macroscopic field U (rho, ux, uy, uz) holds a value that is not finite."""
+    if not np.isfinite(U).all():
+        raise SystemExit(f"[{case}] diverged at step {t}")
+
+
 def _parser():
     import argparse
     ap = argparse.ArgumentParser(description="D3Q19 regularized-BGK + Smagorinsky LES of wind over a dry-stone "
                                              "wall on one MI355X (flags as stonewall3d/lbm3d.py:242-264)")
-    ap.add_argument("--case", required=True, help="output name, e.g. solid / p20 / p40")
-    ap.add_argument("--porosity", type=float, required=True, help="frontal open-area fraction (0 = mortared)")
-    ap.add_argument("--H", type=int, default=40, help="wall height in cells")
-    ap.add_argument("--device", default="cuda", help="cuda / cuda:0")
-    ap.add_argument("--seed", type=int, default=7, help="stone layout seed")
+    ap.add_argument("--case", required=True, help="name of the run; the outputs are <out>/<case>.npz and .ckpt")
+    ap.add_argument("--porosity", type=float, required=True,
+                    help="open fraction of the wall's front face; 0 fills the gaps with mortar")
+    ap.add_argument("--H", type=int, default=40, help="cells across the wall height")
+    ap.add_argument("--device", default="cuda", help="torch device of the buffers (a ROCm GPU)")
+    ap.add_argument("--seed", type=int, default=7, help="seed of the stone layout and the initial noise")
     ap.add_argument("--Re", type=float, default=15000)
     ap.add_argument("--U0", type=float, default=0.08)
     ap.add_argument("--Cs", type=float, default=0.16)
@@ -352,7 +382,7 @@ def _parser():
     ap.add_argument("--t-avg", type=float, default=30)
     ap.add_argument("--snap-dt", type=float, default=0.25)
     ap.add_argument("--snap3d-every", type=int, default=2,
-                    help="store a 3D vortex (Q-criterion) frame every N snapshots (0 = off)")
+                    help="snapshots between two pooled Q-criterion volumes (0 disables them)")
     ap.add_argument("--snap3d-ds", type=int, default=0)
     ap.add_argument("--out", default="results")
     ap.add_argument("--ckpt-every", type=int, default=5000)
@@ -403,11 +433,7 @@ def main(argv=None):
     QMAX, q_scale = 400.0, (H / su.UH) ** 2
     os.makedirs(a.out, exist_ok=True)
     ckpt = os.path.join(a.out, f"{a.case}.ckpt")
-    sig = dict(grid=[nx, ny, nz], H=H, porosity=a.porosity, seed=a.seed, Re=a.Re, U0=a.U0, Cs=a.Cs, n_spin=n_spin,
-               n_total=n_total, snap_every=snap_every, snap3d=[a.snap3d_every, ds3], dtype=a.dtype,
-               ybc=a.ybc, gust_seed=a.gust_seed)
-    if a.dtype in ("f16", "bf16"):
-        sig["shift"] = a.shift
+    sig = run_signature(a, [nx, ny, nz], n_spin, n_total, snap_every, ds3)
     snaps_xz, snaps_xy, snaps_q, t_start = [], [], [], 0
     if os.path.isdir(ckpt) and not a.no_resume:
         t_start, n_acc, snaps = checkpoint.load(sim, ckpt, sig)
@@ -421,7 +447,7 @@ def main(argv=None):
                         dict(snaps_xz=np.array(snaps_xz, np.float16).reshape(-1, 3, nx, nz),
                              snaps_xy=np.array(snaps_xy, np.float16).reshape(-1, 3, nx, ny),
                              snaps_q=np.array(snaps_q, np.uint8).reshape(-1, *q_dims)))
-        print(f"[{a.case}] checkpoint saved at step {t_next}", flush=True)
+        print(f"[{a.case}] step {t_next}: state written to {ckpt}", flush=True)
 
     t0 = time.time()
     for t in range(t_start, n_total):
@@ -446,8 +472,7 @@ def main(argv=None):
             chk = ""
             if t % 5000 == 0:
                 un = sim.macro()
-                if not np.isfinite(un).all():
-                    raise SystemExit(f"[{a.case}] diverged at step {t}")
+                divergence_guard(un, a.case, t)
                 chk = f" rho[{un[0].min():.3f},{un[0].max():.3f}] |u|max/U_H={np.abs(un[1:]).max()/su.UH:.2f}"
             print(f"[{a.case}] step {t}/{n_total}  t={t/su.t_unit:.1f} H/U  {mlups:.0f} MLUPS  "
                   f"eta {el/done*(n_total-t-1)/60:.1f} min{chk}", flush=True)
@@ -470,8 +495,7 @@ def main(argv=None):
         mean_xz_spanavg=np.where(wall.any(1)[None], np.nan, mean.mean(2)).astype(np.float32),
         mean_xy_half=mean[:, :, :, z_half].astype(np.float32),
         mean3d_ds2=mean[1:, ::2, ::2, ::2].astype(np.float16),
-        snaps_xz=np.array(snaps_xz, np.float16).reshape(-1, 3, nx, nz),
-        snaps_xy=np.array(snaps_xy, np.float16).reshape(-1, 3, nx, ny),
+        snaps_xz=np.array(snaps_xz), snaps_xy=np.array(snaps_xy),      # as lbm3d.py:418 stacks them
         wall_xz=wall[:, y_mid, :], wall_xy=wall[:, :, z_half], wall_front=wall.any(0), wall_top=wall.any(2),
         wall_ds2=wall[::2, ::2, ::2], snaps_q=np.array(snaps_q, dtype=np.uint8).reshape(-1, *q_dims),
         meta=json.dumps(dict(case=a.case, porosity_target=a.porosity, porosity=float(phi), H=H, T=T, X0=X0,
@@ -483,10 +507,11 @@ def main(argv=None):
                              gust_seed=a.gust_seed, wall_seconds=time.time() - t0)))
     s = slice(n_spin, None)
     if n_total > n_spin:
-        print(f"[{a.case}] done. porosity={phi:.3f}  Cd mean={(Fx[s]/(qref*A)).mean():.3f} "
-              f"max={(Fx[s]/(qref*A)).max():.3f}  Cm mean={(M[s]/(qref*A*H)).mean():.3f}", flush=True)
+        cd_avg = Fx[s] / (qref * A)
+        print(f"[{a.case}] finished: frontal porosity {phi:.3f}, over the averaging window Cd {cd_avg.mean():.3f} "
+              f"(peak {cd_avg.max():.3f}), Cm {(M[s] / (qref * A * H)).mean():.3f}", flush=True)
     else:
-        print(f"[{a.case}] done (stopped before the averaging window). porosity={phi:.3f}", flush=True)
+        print(f"[{a.case}] finished before the averaging window; frontal porosity {phi:.3f}", flush=True)
     if os.path.isdir(ckpt):
         shutil.rmtree(ckpt)
 

@@ -51,6 +51,7 @@ class Result:
     cd_max: float
     checksum: float
     force: np.ndarray = field(default=None, repr=False)
+    state: object = field(default=None, repr=False)     # final population buffer (device), if kept
 
     @property
     def latency(self):
@@ -83,9 +84,10 @@ class Server:
 
     policy = "fifo"
     fill_cells = None
+    keep_state = False
 
     def __init__(self, budget_bytes=250e9, K=32, max_instances=64, graph=True, keep_force=False, noise=0.02,
-                 device="cuda", variants=None, policy="fifo", fill_cells=None):
+                 device="cuda", variants=None, policy="fifo", fill_cells=None, keep_state=False):
         if policy not in ("fifo", "sjf"):
             raise ValueError("policy must be 'fifo' or 'sjf'")
         self.policy = policy
@@ -95,6 +97,7 @@ class Server:
         self.max_instances = int(max_instances)
         self.graph = bool(graph)
         self.keep_force = keep_force
+        self.keep_state = keep_state
         self.noise = float(noise)
         self.device = torch.device(device)
         self.variants = dict(variants or {})
@@ -129,7 +132,7 @@ class Server:
         cs = float(np.sum(F))
         return Result(req.rid, req.dtype, req.H, lat.N, req.steps, req.arrival, lat.admitted - t0,
                       time.perf_counter() - t0, float(cd.mean()), float(cd.max()), cs,
-                      F if self.keep_force else None)
+                      F if self.keep_force else None, lat.current().clone() if self.keep_state else None)
 
     def tune(self, dtypes=("f32", "f16"), heights=(16, 24, 32), tuner=None):
         """Pick each dtype's tile variant and persistent grid with the autotuner, on a
