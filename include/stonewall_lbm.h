@@ -173,6 +173,21 @@ int sw_selftest_division(unsigned long long* counts, void* stream);
  * against (bench.py), and it is the known-size kernel that scales the PMC byte counters. */
 int sw_stream_copy(const void* src, void* dst, long long bytes, void* stream);
 
+/* Allocation-free step.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  It is sw_step followed by sw_force_reduce of that one step, with
+ * the FP64 force partials in a fixed device array of the library (4 MiB per GPU, held until
+ * the process ends) instead of a caller's ring; it allocates nothing.  The caller passes
+ * inputs and outputs as for sw_step, and `force` is the [T][nz][2] FP64 force record whose
+ * row t (+ *t_dev) is written.  Lattices with nz * ceil(ny/4) > sw_noalloc_max_pairs()
+ * (262,144) are refused with SW_ERR_TOO_LARGE.  All calls on one GPU share that GPU's array:
+ * calls that overlap on one GPU (different streams without ordering) race. */
+long long sw_noalloc_max_pairs(void);
+int sw_step_noalloc(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
+                    const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
+                    int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* force,
+                    void* stream);
+
 /* ---------------------------------------------------------------- batched form
  * No reference counterpart: the reference runs one lattice per process (run_all.sh).
  * Many independent lattices of one dtype advance together: one persistent launch per

@@ -39,6 +39,9 @@ SIGNATURES = {
     "sw_qcoarse": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "sw_selftest_division": (_c_int, [_p, _p]),
     "sw_stream_copy": (_c_int, [_p, _p, _c_ll, _p]),
+    "sw_noalloc_max_pairs": (_c_ll, []),
+    "sw_step_noalloc": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
+                                 _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _p]),
     "sw_batch_table_bytes": (_c_ll, [_c_int]),
     "sw_batch_upload": (_c_int, [_p, _c_int, _c_int, _p, _p]),
     "sw_batch_download": (_c_int, [_p, _p, _c_int, _p]),
