@@ -106,6 +106,23 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
             int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
             int ring, void* stream);
 
+/* Two fused steps per launch: f_in holds step t, f_out receives step t + 2; the state of
+ * step t + 1 stays on chip (no reference counterpart: the reference launches k_step once
+ * per step, lbm3d.py:359-361; this is two of those launches with one HBM pass).  Arguments
+ * as sw_step; populations are bit-identical to sw_step(t) followed by sw_step(t + 1).
+ * FP32, FP16 and BF16 storage with population arrays below 4 GiB.  The gust table needs
+ * entries t and t + 1.  partials: ring * sw_step2_partials_per_step(nx, ny, nz) doubles
+ * (ring >= 2), slots t % ring and (t + 1) % ring; reduce with sw_force_reduce2.  The force
+ * partials are per 64-cell x tile and tile row block, so force rows equal sw_step's to
+ * rounding (a different fixed summation order), not bit for bit. */
+long long sw_step2_partials_per_step(int nx, int ny, int nz);
+int sw_step2(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
+             int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
+             int ybc, float scale, const float* shift, double* partials, int ring, void* stream);
+/* force[t][z][0..1] from the sw_step2 partials ring, as sw_force_reduce does for sw_step's. */
+int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int nx, int ny, int nz,
+                     double* force, void* stream);
+
 /* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
  * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept
  * per dtype for the process; every variant gives bit-identical populations.
@@ -167,8 +184,9 @@ int sw_qcoarse(const float* U, const uint32_t* links, int nx, int ny, int nz, in
  * 2^32 hashed samples (|a| in [2^-40, 4), b in [2^-4, 4)).  Both must be 0. */
 int sw_selftest_division(unsigned long long* counts, void* stream);
 
-/* Streaming copy dst = src of `bytes` (a multiple of 16; both 16-byte aligned): 16-byte
- * loads and nontemporal 16-byte stores, eight 256-thread workgroups per compute unit.  No
+/* Streaming copy dst = src of `bytes` (a multiple of 16; both 16-byte aligned): contiguous
+ * 16 KB blocks per workgroup, 16-byte loads and nontemporal 16-byte stores, 32 workgroups
+ * per compute unit.  No
  * reference counterpart: it measures the HBM read + write ceiling the step is compared
  * against (bench.py), and it is the known-size kernel that scales the PMC byte counters. */
 int sw_stream_copy(const void* src, void* dst, long long bytes, void* stream);

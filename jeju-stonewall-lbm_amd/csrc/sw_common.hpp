@@ -96,7 +96,9 @@ template <> struct Codec<_Float16> {
     // high half (v_fma_mixhi_f16).
     using Pending = half2_t;
     static __device__ __forceinline__ Pending first(float v, float ns, float scale) {
-        Pending p = {};
+        // the high half is left unset: second() writes it before the word is used (a zero
+        // fill here cost one v_pack per direction)
+        Pending p;
         p.x = (_Float16)__builtin_fmaf(v, scale, ns);
         return p;
     }

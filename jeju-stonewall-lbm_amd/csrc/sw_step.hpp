@@ -16,6 +16,7 @@
 // are bit-reproducible (the reference uses FP32 atomics, whose order is not).
 #pragma once
 #include "sw_common.hpp"
+#include <type_traits>
 
 namespace sw {
 
@@ -30,6 +31,7 @@ struct StepArgs {
     double* partials;        // [ring][nz][nyb][2], or null
     const int* t_dev;        // optional device step counter added to t
     const float* shift;      // 16-bit storage: [nz][4] offsets sigma (sw_shift_table), or null (sigma = 1)
+    float* acc;              // [4][N] accumulator += (rho, ux, uy, uz) of the step's output (ACC kernels)
     int t;
     int ring;
     int nx, ny, nz, nyb;
@@ -207,6 +209,37 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const Step
     });
 }
 
+// k_macro (lbm3d.py:158-182) of one cell's stored populations sv[] (decoded as a following
+// step or sw_macro reads them), added into the accumulator as k_accum does (:185-188):
+// sequential sums in direction order (FP32 / FP64) or the opposite-pair moments of the
+// 16-bit forms, u = m / rho by IEEE division, acc[k] = acc[k] + u[k].
+template <bool NORM, class T>
+__device__ __forceinline__ void macro_accumulate(float* acc, size_t N, size_t cell, const T (&sv)[Q]) {
+    T rho = T(0), mx = T(0), my = T(0), mz = T(0);
+    if constexpr (NORM) {
+        const Moments<T> m = moments_norm<T, false>(sv);
+        rho = m.rho; mx = m.mx; my = m.my; mz = m.mz;
+    } else {
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            const T v = sv[i];
+            rho = rho + v;
+            if constexpr (CX[i] != 0) mx = CX[i] > 0 ? mx + v : mx - v;
+            if constexpr (CY[i] != 0) my = CY[i] > 0 ? my + v : my - v;
+            if constexpr (CZ[i] != 0) mz = CZ[i] > 0 ? mz + v : mz - v;
+        });
+    }
+    const float u[4] = {(float)rho, (float)(mx / rho), (float)(my / rho), (float)(mz / rho)};
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k * N + cell] = acc[k * N + cell] + u[k];
+}
+// a solid cell's sample: (1, 0, 0, 0) as k_macro gives it
+__device__ __forceinline__ void solid_accumulate(float* acc, size_t N, size_t cell) {
+    const float u[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k * N + cell] = acc[k * N + cell] + u[k];
+}
+
 // Number of 32-bit words of chunk flags per row: bit c of a row's words is set when the
 // 64-cell chunk c holds a solid cell or a cell with a solid neighbour (x = 0 excluded).
 // The words of all rows follow the N link masks in the links buffer.
@@ -265,7 +298,9 @@ constexpr int NUM_TILE_VARIANTS = 14;
 
 // One (y, z) row of one lattice; adds this lane's FP64 wall-force sums to fx_acc/fz_acc.
 // BUF selects raw-buffer addressing (all byte offsets of a population array < 4 GiB).
-template <class S, class TL, bool BUF>
+// ACC: also add (rho, u) of every cell's output state into a.acc (k_macro + k_accum on the
+// step's result, lbm3d.py:363-370), without re-reading the populations.
+template <class S, class TL, bool BUF, bool ACC = false>
 __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t, int lane, int seg,
                                          double& fx_acc, double& fz_acc) {
     using C = Codec<S>;
@@ -279,7 +314,11 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     const size_t NB = (size_t)N * ES;                            // bytes per population
     const uint32_t rowid = (uint32_t)z * ny + y;
     const uint32_t selfrow = rowid * nx;
-    const size_t selfb = (size_t)selfrow * ES;
+    // Byte offsets of rows: 32-bit with raw-buffer addressing (every offset of the array is
+    // below 4 GiB), which halves the scalar registers the 19 row offsets take.
+    using Off = std::conditional_t<BUF, uint32_t, size_t>;
+    const Off selfb = (Off)((size_t)selfrow * ES);
+    const Off NBo = (Off)NB;
     T inv_scale = (T)a.inv_scale, scale = (T)a.scale;
     // 16-bit decodes fma(h, inv_scale, sigma): inv_scale in a vector register, sigma scalar.
     // (A vector copy of the encode scale as well costs a VGPR: 73, one wavefront per SIMD less.)
@@ -292,7 +331,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     // wave-uniform byte offset of the source row of every link after the y / z boundary
     // rules, pre-shifted by -cx so that every load of a cell uses the lane offset x*ES
     // (cx = -1 loads use the outlet-clamped lane offset instead: lbm3d.py:82-83)
-    size_t srow[Q];
+    Off srow[Q];
     // 16-bit forms: the stored value of direction i at height z is (phi_i - sigma_c(z)) * scale,
     // phi_i = f_i / w_i, with sigma depending on the direction only through its c_x class
     // (0, +1, -1).  Per row this needs sigma of the three classes at heights z-1, z, z+1
@@ -337,7 +376,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         }
         if (zs >= (int)nz) { zs = z; j = MIRZ[j]; }                               // lbm3d.py:84-86
         const size_t r = (size_t)j * NB + (size_t)(((uint32_t)zs * ny + (uint32_t)ys) * nx) * ES;
-        srow[i] = (CX[i] > 0) ? r - ES : r;
+        srow[i] = (Off)((CX[i] > 0) ? r - ES : r);
     });
     const uint32_t* lrow = a.links + selfrow;
     const uint32_t cw = chunk_words(nx);
@@ -360,7 +399,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         const uint32_t xo = x * ES;
         collide<NORM, T>(fl, x, a, smag, [&](auto I, T v) {
             constexpr int i = I;
-            mem.template st<TL::NT>((size_t)i * NB + selfb, xo, C::encode(v, esh(I), scale));
+            mem.template st<TL::NT>((Off)i * NBo + selfb, xo, C::encode(v, esh(I), scale));
         });
     };
     // general path: per-link bounce-back and wall force (:77-101).  Each lane picks the
@@ -483,7 +522,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             uint32_t rl[Q], rh[Q];             // cells (xq, xq+1) and (xq+2, xq+3), 16 bits each
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
-                const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
+                const Off row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
                 const uint64_t cur = mem.ld64(row, lb);
                 const uint32_t lo = (uint32_t)cur, hi = (uint32_t)(cur >> 32);
                 if constexpr (CX[i] == 0) {
@@ -529,7 +568,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             if (xq < nx) {
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    mem.template st64<TL::NT>((size_t)i * NB + selfb, xq * 2, (uint64_t)ol[i] | ((uint64_t)oh[i] << 32));
+                    mem.template st64<TL::NT>((Off)i * NBo + selfb, xq * 2, (uint64_t)ol[i] | ((uint64_t)oh[i] << 32));
                 });
             }
         }
@@ -579,7 +618,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             const uint32_t redge = (x0 + 128 < nx ? x0 + 128 : x0) * 2;   // dword holding cell x0+128
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
-                const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
+                const Off row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
                 cur[i] = mem.template ld32<TL::NTL>(row, lb);
                 if constexpr (CX[i] < 0) edge[i] = mem.ld32(row, redge);
             });
@@ -642,7 +681,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             if (xa < nx) {
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    mem.template st32<TL::NT>((size_t)i * NB + selfb, xa * 2, out[i]);
+                    mem.template st32<TL::NT>((Off)i * NBo + selfb, xa * 2, out[i]);
                 });
             }
         };
@@ -715,7 +754,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             T fa[Q], fb[Q];
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
-                const size_t row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
+                const Off row = (CX[i] > 0) ? srow[i] + ES : srow[i];   // unshifted row start
                 const uint64_t cur = mem.template ld64<TL::NTL>(row, lb);
                 const uint32_t lo = (uint32_t)cur, hi = (uint32_t)(cur >> 32);
                 uint32_t va, vb;
@@ -752,7 +791,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             if (xa < nx) {
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    mem.template st64<TL::NT>((size_t)i * NB + selfb, xa * 4, ob[i]);
+                    mem.template st64<TL::NT>((Off)i * NBo + selfb, xa * 4, ob[i]);
                 });
             }
             carried = true;
@@ -796,7 +835,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         const T usq15 = T(1.5) * (u0 * u0);
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            mem.template st<false>((size_t)i * NB + selfb, 0,
+            mem.template st<false>((Off)i * NBo + selfb, 0,
                                    C::encode(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale));
         });
     }

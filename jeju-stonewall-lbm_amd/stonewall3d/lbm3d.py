@@ -88,10 +88,14 @@ class Simulation:
     gust      : optional (n_steps,) float32 inlet factor per step.
     n_steps   : steps this run may take (sizes the force record and the gust table).
     ring      : steps per force-partials ring (reduced every `ring` steps).
+    two_step  : advance pairs of steps with one launch of the two-step kernel (sw_step2:
+                FP32 / FP16 / BF16 storage, population arrays below 4 GiB); populations are
+                bit-identical to single steps, force rows equal to rounding.
     """
 
     def __init__(self, flag, uin, *, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, ybc="periodic", dtype="f32",
-                 scale=None, gust=None, n_steps=1000, ring=64, device="cuda", record_force=True, shift=None):
+                 scale=None, gust=None, n_steps=1000, ring=64, device="cuda", record_force=True, shift=None,
+                 two_step=False):
         nat.lib()
         flag = np.asarray(flag)
         if flag.dtype == bool:
@@ -134,6 +138,17 @@ class Simulation:
         per = nat.lib().sw_partials_per_step(self.ny, self.nz)
         self.record_force = bool(record_force)
         self.partials = torch.zeros(self.ring * per, dtype=torch.float64, device=dev) if record_force else None
+        self.two_step = bool(two_step)
+        if self.two_step:
+            if self.dtype == nat.SW_F64 or Q * self.N * nat.DTYPE_BYTES[self.dtype] >= (1 << 32):
+                raise ValueError("two_step needs FP32 / FP16 / BF16 storage and population arrays below 4 GiB")
+            if self.ring < 2:
+                raise ValueError("two_step needs ring >= 2")
+        self.partials2 = None
+        if self.two_step and record_force:
+            per2 = nat.lib().sw_step2_partials_per_step(self.nx, self.ny, self.nz)
+            self.partials2 = torch.zeros(self.ring * per2, dtype=torch.float64, device=dev)
+        self._layout = 1                # kernel that wrote the pending partials: 1 = sw_step, 2 = sw_step2
         self.force = torch.zeros((self.n_steps, self.nz, 2), dtype=torch.float64, device=dev) if record_force else None
         self.t = 0
         self._pending = 0
@@ -155,21 +170,36 @@ class Simulation:
         return self
 
     def step(self, n=1):
-        """n fused steps (lbm3d.py:358-362)."""
+        """n fused steps (lbm3d.py:358-362); with two_step, pairs of them per launch."""
         lib = nat.lib()
         s = _stream()
-        for _ in range(int(n)):
-            if self.t >= self.n_steps:
-                raise RuntimeError(f"step {self.t} beyond n_steps={self.n_steps}")
-            rc = lib.sw_step(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
-                             self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
-                             self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
-                             _ptr(self.shift), _ptr(self.partials), self.ring, s)
-            nat.check(rc, "sw_step")
+        n = int(n)
+        if self.t + n > self.n_steps:
+            raise RuntimeError(f"step {self.t + n - 1} beyond n_steps={self.n_steps}")
+        while n > 0:
+            pair = self.two_step and n >= 2
+            layout = 2 if pair else 1
+            if self.record_force and self._pending and (self._layout != layout or
+                                                        self._pending + layout > self.ring):
+                self.flush_force()
+            self._layout = layout
+            if pair:
+                rc = lib.sw_step2(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
+                                  self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
+                                  self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
+                                  _ptr(self.shift), _ptr(self.partials2), self.ring, s)
+                nat.check(rc, "sw_step2")
+            else:
+                rc = lib.sw_step(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
+                                 self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
+                                 self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
+                                 _ptr(self.shift), _ptr(self.partials), self.ring, s)
+                nat.check(rc, "sw_step")
             self.f[0], self.f[1] = self.f[1], self.f[0]
-            self.t += 1
+            self.t += layout
+            n -= layout
             if self.record_force:
-                self._pending += 1
+                self._pending += layout
                 if self._pending == self.ring:
                     self.flush_force()
         return self
@@ -177,8 +207,12 @@ class Simulation:
     def flush_force(self):
         """Reduce the pending partials into force rows (fixed summation order)."""
         if self.record_force and self._pending:
-            nat.call("sw_force_reduce", self.partials.data_ptr(), self.ring, self.t - self._pending, None,
-                     self._pending, self.ny, self.nz, self.force.data_ptr(), _stream())
+            if self._layout == 2:
+                nat.call("sw_force_reduce2", self.partials2.data_ptr(), self.ring, self.t - self._pending, None,
+                         self._pending, self.nx, self.ny, self.nz, self.force.data_ptr(), _stream())
+            else:
+                nat.call("sw_force_reduce", self.partials.data_ptr(), self.ring, self.t - self._pending, None,
+                         self._pending, self.ny, self.nz, self.force.data_ptr(), _stream())
             self._pending = 0
 
     # ------------------------------------------------------------------ readout
