@@ -2,24 +2,28 @@
 
 Workload (default): the H=40 porous stone wall at 20 % void fraction, 640 x 320 x 160
 cells (32.8 M), gusting inlet, nu = 2e-4, Cs = 0.16, free-slip top and sides, FP32
-populations.  A step is one fused kernel launch over the whole lattice, plus every 10
-steps the FP64 force-row reduction (inside the timed region).  Inputs are resident in
-HBM before timing starts.
+populations.  A step is one update of the whole lattice by the fused step; FP32 runs two
+steps per launch (k_step2: one HBM pass for two updates, bit-identical to two k_step
+launches), the 16-bit forms one step per launch (k_step); every 10 steps the FP64
+force-row reduction runs inside the timed region.  Inputs are resident in HBM before
+timing starts.
 
 Prints ONE JSON line:
   value / ms_per_step   MLUPS over all nx*ny*nz cells (as lbm3d.py:384 counts them) from
                         the HIP-event time of the whole timed region
   roofline              achieved = algorithmic bytes (152 B per FP32 cell update: 19 loads
                         + 19 stores of 4 B; 76 B for the 16-bit forms) / ms_per_step, against
-                        8 TB/s; kernel_avg_ms is the step kernel's mean launch duration
-                        (events around each launch); traffic is the HBM bytes per launch from
-                        the rocprofv3 PMC summary in profiles/ (FETCH_SIZE and WRITE_SIZE,
-                        scaled by the same counters on a copy of known size)
+                        8 TB/s; kernel_avg_ms is the main kernel's mean launch duration
+                        (events around each launch, steps_per_launch updates each); traffic
+                        is the HBM bytes per launch from the rocprofv3 PMC summary in
+                        profiles/ (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a
+                        copy of known size)
+  fp32_one_step         the one-step-per-launch FP32 kernel timed the same way
   copy_ceiling          the same process's streaming copy of one population array (the
                         read + write pattern of a step without arithmetic), and the step's
                         fraction of it
   fp16_storage          the FP16-storage form timed the same way on the same lattice after
-                        the FP32 run, with its ratio to the FP32 MLUPS
+                        the FP32 run, with its ratio to the FP32 MLUPS (headline and one-step)
   cpu_baseline          the FP32 oracle (a C restatement of the reference kernel) on the
                         host: a bounded sample of the same lattice on all cores (value) and
                         on one thread, plus the 160 x 80 x 40 box-wall case
@@ -38,10 +42,10 @@ METRIC = "MLUPS and % HBM roofline (FP32 vs FP16-storage, Cd err vs FP64); p50/p
 HBM_PEAK_GBS = 8000.0
 
 
-def traffic_from_profile(dtype):
-    """HBM bytes per step launch and the summary they come from (rocprofv3 PMC counters
-    FETCH_SIZE / WRITE_SIZE of k_step on the 640x320x160 lattice), if the summary exists."""
-    name = f"k_step_porous_wall_640x320x160_{dtype}_traffic.json"
+def traffic_from_profile(dtype, kernel="k_step"):
+    """HBM bytes per launch of `kernel` and the summary they come from (rocprofv3 PMC
+    counters FETCH_SIZE / WRITE_SIZE on the 640x320x160 lattice), if the summary exists."""
+    name = f"{kernel}_porous_wall_640x320x160_{dtype}_traffic.json"
     p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
@@ -94,31 +98,40 @@ def cpu_baseline(w, budget_s=10.0):
                 box_wall_160x80x40=dict(all_cores=c0_all, single_thread=c0_one))
 
 
-def timed_steps(w, dtype, shift, warmup, steps, ring=10):
+def two_step_ok(w, dtype):
+    """The two-steps-per-launch kernel takes FP32 / FP16 / BF16 arrays below 4 GiB."""
+    return dtype != "f64" and 19 * w.cells * (4 if dtype == "f32" else 2) < (1 << 32)
+
+
+def timed_steps(w, dtype, shift, warmup, steps, ring=10, two_step=False):
     """Build the lattice in HBM, take `warmup` untimed steps, then time exactly `steps` steps:
     HIP events around the whole run and around every launch, on the launch stream (the
-    force-row reduction runs every `ring` steps inside the timed region).
-    Returns (simulation, total ms, mean ms per step launch)."""
+    force-row reduction runs every `ring` steps inside the timed region).  With two_step,
+    each launch advances two steps (an odd count ends with one single step).
+    Returns (simulation, total ms, mean ms per launch of the main kernel, steps per launch)."""
     import numpy as np
     import torch
     from stonewall3d.lbm3d import Simulation
-    sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, ring=ring, **w.sim_kwargs())
+    sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, ring=ring, two_step=two_step, **w.sim_kwargs())
     sim.init(noise=0.02, seed=0)
     sim.step(warmup)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    per = 2 if two_step else 1
+    chunks = [per] * (steps // per) + ([steps % per] if steps % per else [])
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in chunks]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_start.record(stream)
-    for k in range(steps):
-        ev[k][0].record(stream)
-        sim.step(1)
-        ev[k][1].record(stream)
+    for (a, b), n in zip(ev, chunks):
+        a.record(stream)
+        sim.step(n)
+        b.record(stream)
     t_end.record(stream)
     torch.cuda.synchronize()
-    return sim, t_start.elapsed_time(t_end), float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    main = [a.elapsed_time(b) for (a, b), n in zip(ev, chunks) if n == per] or [a.elapsed_time(b) for a, b in ev]
+    return sim, t_start.elapsed_time(t_end), float(np.mean(main)), per
 
 
 def copy_ceiling(nbytes, reps=10):
@@ -160,6 +173,8 @@ def main():
                     help="time the steps of --dtype only: no CPU baseline, no copy ceiling and no FP16-storage "
                          "comparison leg (the form the rocprofv3 runs use)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--one-step", action="store_true",
+                    help="time the one-step-per-launch kernel (k_step) instead of two steps per launch (k_step2)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only the streaming copy of one FP32 population array (19 x 640 x 320 x 160 x 4 B), 12 "
                          "times: the known-size copy against which rocprofv3's FETCH_SIZE / WRITE_SIZE are scaled")
@@ -191,7 +206,11 @@ def main():
                 f"free-slip top/sides")
     nx, ny, nz = w.shape
     N = nx * ny * nz
-    sim, total_ms, kern_ms = timed_steps(w, a.dtype, a.shift, a.warmup, a.steps)
+    # the fastest kernel of each storage form on this lattice: two steps per launch for FP32
+    # (0.79 against 0.65 of 8 TB/s), one step per launch for the 16-bit forms (the two-step
+    # form of those is limited by registers and VALU and runs at about half their one-step rate)
+    two = (a.dtype == "f32" and not a.one_step and two_step_ok(w, a.dtype))
+    sim, total_ms, kern_ms, per = timed_steps(w, a.dtype, a.shift, a.warmup, a.steps, two_step=two)
     F = sim.force_rows()
     if not np.isfinite(F).all():
         raise SystemExit("force rows are not finite")
@@ -201,7 +220,8 @@ def main():
     bpu = bytes_per_update(a.dtype)
     achieved = N * bpu / (ms_step * 1e-3) / 1e9
     headline = a.workload == "porous_wall" and a.H == 40 and a.porosity == 0.2
-    traffic, traffic_src = traffic_from_profile(a.dtype) if headline else (None, None)
+    traffic, traffic_src = traffic_from_profile(a.dtype, "k_step2" if per == 2 else "k_step") if headline \
+        else (None, None)
 
     line = {
         "metric": METRIC,
@@ -222,8 +242,9 @@ def main():
                    "storage_offset": sim.shift_kind if a.dtype in ("f16", "bf16") else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_step", "kernel_avg_ms": round(kern_ms, 4),
-                     "kernel_frac": round(N * bpu / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "kernel": "k_step2" if per == 2 else "k_step", "steps_per_launch": per,
+                     "kernel_avg_ms": round(kern_ms, 4),
+                     "kernel_frac": round(per * N * bpu / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "fluid_mlups": round(fluid / (ms_step * 1e-3) / 1e6, 1),
     }
     del sim
@@ -235,9 +256,19 @@ def main():
                                 "step_frac_of_copy": round(achieved / gbs, 4), "ms": round(cms, 4),
                                 "what": f"sw_stream_copy of one FP32 population array ({pop_bytes / 1e9:.2f} GB), "
                                         "bytes read + written per second"}
+    fp32_one = None
+    if not a.no_cpu_baseline and a.dtype == "f32" and per == 2:
+        # the one-step-per-launch FP32 kernel on the same lattice, for comparison
+        s1, t1, k1, _ = timed_steps(w, "f32", a.shift, a.warmup, a.steps)
+        fp32_one = N / (t1 / a.steps * 1e-3) / 1e6
+        line["fp32_one_step"] = {"value": round(fp32_one, 1), "unit": "MLUPS", "kernel": "k_step",
+                                 "ms_per_step": round(t1 / a.steps, 4), "kernel_avg_ms": round(k1, 4),
+                                 "roofline_frac": round(N * bpu / (t1 / a.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del s1
+        torch.cuda.empty_cache()
     if not a.no_cpu_baseline and a.dtype == "f32":
         # the metric's FP16-storage side, on the same lattice and step count (outside the timed FP32 region)
-        s16, t16, k16 = timed_steps(w, "f16", a.shift, a.warmup, a.steps)
+        s16, t16, k16, _ = timed_steps(w, "f16", a.shift, a.warmup, a.steps)
         if not np.isfinite(s16.force_rows()).all():
             raise SystemExit("FP16 force rows are not finite")
         ms16 = t16 / a.steps
@@ -245,6 +276,8 @@ def main():
         a16 = N * bytes_per_update("f16") / (ms16 * 1e-3) / 1e9
         tr16, src16 = traffic_from_profile("f16") if headline else (None, None)
         line["fp16_storage"] = {"value": round(m16, 1), "unit": "MLUPS", "ratio_vs_fp32": round(m16 / mlups, 3),
+                                "ratio_vs_fp32_one_step": round(m16 / fp32_one, 3) if fp32_one else None,
+                                "kernel": "k_step",
                                 "ms_per_step": round(ms16, 4), "kernel_avg_ms": round(k16, 4),
                                 "bytes_per_update": bytes_per_update("f16"), "achieved_gbs": round(a16, 1),
                                 "roofline_frac": round(a16 / HBM_PEAK_GBS, 4),

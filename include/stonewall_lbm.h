@@ -106,6 +106,16 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
             int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
             int ring, void* stream);
 
+/* sw_step that also adds (rho, ux, uy, uz) of every cell of the new state (f_out) into the
+ * accumulator acc[4][N] (float, this library's [c][z][y][x] layout), exactly as sw_macro(f_out,
+ * Uacc = acc) after the step would (solid cells add (1, 0, 0, 0)), without reading the
+ * populations again.  Replaces the averaging launches k_macro + k_accum that follow k_step
+ * on sampled steps, lbm3d.py:363-370. */
+int sw_step_accumulate(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
+                       const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
+                       int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
+                       int ring, float* acc, void* stream);
+
 /* Two fused steps per launch: f_in holds step t, f_out receives step t + 2; the state of
  * step t + 1 stays on chip (no reference counterpart: the reference launches k_step once
  * per step, lbm3d.py:359-361; this is two of those launches with one HBM pass).  Arguments

@@ -58,7 +58,8 @@ inline int check_dtype(int dtype, float scale) {
 // argument checks (sw_kernels.hip); other entry points of the library reuse them.
 int step_entry(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
                int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
-               int ybc, float scale, const float* shift, double* partials, int ring, hipStream_t s);
+               int ybc, float scale, const float* shift, double* partials, int ring, hipStream_t s,
+               float* acc = nullptr);
 int force_reduce_entry(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int ny, int nz,
                        double* force, hipStream_t s);
 

@@ -5,7 +5,7 @@
 namespace sw {
 
 // ------------------------------------------------------------------ step
-template <class S, class TL, bool BUF>
+template <class S, class TL, bool BUF, bool ACC = false>
 __global__ __launch_bounds__(64 * TL::WPB, TL::OCC) void k_step(StepArgs a) {
     // the wave index is uniform per wavefront; readfirstlane lets the compiler keep every
     // row-derived pointer in scalar registers
@@ -15,7 +15,7 @@ __global__ __launch_bounds__(64 * TL::WPB, TL::OCC) void k_step(StepArgs a) {
     const int y = yb * ROWS + (wave % ROWS);
     const int t = a.t + (a.t_dev ? *a.t_dev : 0);
     double fx = 0.0, fz = 0.0;
-    if (y < a.ny) step_row<S, TL, BUF>(a, y, z, t, lane, wave / ROWS, fx, fz);
+    if (y < a.ny) step_row<S, TL, BUF, ACC>(a, y, z, t, lane, wave / ROWS, fx, fz);
     block_force<TL::WPB>(a, fx, fz, z, yb, t);
 }
 
@@ -328,10 +328,10 @@ using namespace sw;
 // Lattices whose byte offsets exceed 32 bits use 64-bit addressing with nontemporal stores.
 static int g_step_variant[4] = {9, 9, 9, 1};   // measured best on MI355X, 640x320x160 (f32, f16, bf16, f64)
 
-template <class S, class TL, bool BUF>
+template <class S, class TL, bool BUF, bool ACC = false>
 static void launch_step_v(const StepArgs& a, hipStream_t s) {
     dim3 grid(a.nz * a.nyb), blk(64 * TL::WPB);
-    hipLaunchKernelGGL((k_step<S, TL, BUF>), grid, blk, 0, s, a);
+    hipLaunchKernelGGL((k_step<S, TL, BUF, ACC>), grid, blk, 0, s, a);
 }
 
 // raw-buffer addressing needs every byte offset of a population array below 4 GiB
@@ -342,8 +342,22 @@ static bool fits_buffer(const StepArgs& a, size_t elem) {
 template <class S>
 static void launch_step(const StepArgs& a, int dtype, hipStream_t s) {
     // 64-bit addressing (arrays of 4 GiB and more): plain sweep with nontemporal stores
-    if (!fits_buffer(a, sizeof(S))) { launch_step_v<S, TileV1, false>(a, s); return; }
+    if (!fits_buffer(a, sizeof(S))) {
+        if (a.acc) launch_step_v<S, TileV1, false, true>(a, s);
+        else launch_step_v<S, TileV1, false>(a, s);
+        return;
+    }
     int v = g_step_variant[dtype];
+    if (a.acc) {
+        // the accumulate epilogue is built for the pair tiles (16/32-bit storage) and the
+        // one-cell tile; every tile gives the same populations
+        if (sizeof(S) == 8 || (v != 8 && v != 9)) v = sizeof(S) == 8 ? 1 : 9;
+        if (v == 9 && (sizeof(S) == 2 ? a.nx < 512 : a.nx <= 128)) v = 8;
+        if (v == 1) launch_step_v<S, TileV1, true, true>(a, s);
+        else if (v == 8) launch_step_v<S, TileV8, true, true>(a, s);
+        else launch_step_v<S, TileV9, true, true>(a, s);
+        return;
+    }
     // Variant 9 splits each row's 128-cell pair chunks over two wavefronts; with only a few
     // chunks per row one of them idles or gets the smaller share, and one wavefront per row
     // (variant 8) is faster: measured for 16-bit rows below 512 cells and FP32 rows of one
@@ -363,7 +377,7 @@ namespace sw {
 
 int step_entry(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
                int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
-               int ybc, float scale, const float* shift, double* partials, int ring, hipStream_t s) {
+               int ybc, float scale, const float* shift, double* partials, int ring, hipStream_t s, float* acc) {
     SW_CHECK_ARG(f_in && f_out && links && uin, "f_in, f_out, links and uin must be non-null");
     SW_CHECK_ARG(f_in != f_out, "f_in and f_out must be different buffers (pull scheme)");
     SW_CHECK_RC(check_dims(nx, ny, nz));
@@ -378,6 +392,7 @@ int step_entry(const void* f_in, void* f_out, int dtype, const uint32_t* links, 
     a.nx = nx; a.ny = ny; a.nz = nz; a.nyb = (ny + ROWS - 1) / ROWS;
     a.x_sponge = x_sponge; a.ybc = ybc; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
     a.scale = scale; a.inv_scale = 1.0f / scale;
+    a.acc = acc;
     switch (dtype) {
         case DT_F32: launch_step<float>(a, dtype, s); break;
         case DT_F64: launch_step<double>(a, dtype, s); break;
@@ -468,6 +483,15 @@ int sw_step(const void* f_in, void* f_out, int dtype, const uint32_t* links, con
             int ybc, float scale, const float* shift, double* partials, int ring, void* stream) {
     return step_entry(f_in, f_out, dtype, links, uin, gust, t, t_dev, nx, ny, nz, nu0, cs, x_sponge, nu_sponge, ybc,
                       scale, shift, partials, ring, (hipStream_t)stream);
+}
+
+int sw_step_accumulate(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
+                       const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
+                       int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
+                       int ring, float* acc, void* stream) {
+    SW_CHECK_ARG(acc, "acc must be non-null");
+    return step_entry(f_in, f_out, dtype, links, uin, gust, t, t_dev, nx, ny, nz, nu0, cs, x_sponge, nu_sponge, ybc,
+                      scale, shift, partials, ring, (hipStream_t)stream, acc);
 }
 
 int sw_step_variants(void) { return NUM_TILE_VARIANTS; }

@@ -395,12 +395,28 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             fl[i] = C::decode(mem.template ld<TL::NTL>(srow[i], (CX[i] < 0) ? xpo : xo), dsh(I), inv_scale);
         });
     };
+    auto osh = [&](auto I) -> T {                     // decode offset of the cell's own population I
+        constexpr int i = I;
+        if constexpr (NORM) return sg[1][xclass(CX[i])];
+        else return wgt<T>(i);
+    };
     auto collide_store = [&](uint32_t x, const T (&fl)[Q]) {
         const uint32_t xo = x * ES;
-        collide<NORM, T>(fl, x, a, smag, [&](auto I, T v) {
-            constexpr int i = I;
-            mem.template st<TL::NT>((Off)i * NBo + selfb, xo, C::encode(v, esh(I), scale));
-        });
+        if constexpr (ACC) {
+            T sv[Q];                                   // the stored values as sw_macro decodes them
+            collide<NORM, T>(fl, x, a, smag, [&](auto I, T v) {
+                constexpr int i = I;
+                const typename C::B e = C::encode(v, esh(I), scale);
+                mem.template st<TL::NT>((Off)i * NBo + selfb, xo, e);
+                sv[i] = C::decode(e, osh(I), inv_scale);
+            });
+            macro_accumulate<NORM, T>(a.acc, N, (size_t)selfrow + x, sv);
+        } else {
+            collide<NORM, T>(fl, x, a, smag, [&](auto I, T v) {
+                constexpr int i = I;
+                mem.template st<TL::NT>((Off)i * NBo + selfb, xo, C::encode(v, esh(I), scale));
+            });
+        }
     };
     // general path: per-link bounce-back and wall force (:77-101).  Each lane picks the
     // address of every link (streamed source, or its own opposite population on a
@@ -421,7 +437,10 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
             else off = ((mm >> i) & 1u) ? (size_t)OPP[i] * NB + selfb + xo : srow[i] + ((CX[i] < 0) ? xpo : xo);
             raw[i] = mem.ld_at(off);
         });
-        if (!live) return;
+        if (!live) {
+            if constexpr (ACC) { if (in_row) solid_accumulate(a.acc, N, (size_t)selfrow + x); }   // solid cell
+            return;
+        }
         T fl[Q];
         T fxw = T(0), fzw = T(0);
         bool hit = false;
@@ -477,6 +496,7 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
     };
 
     bool inlet_done = false;
+    static_assert(!(ACC && TL::QUAD), "the accumulate epilogue runs on the one-cell and pair tiles");
     if constexpr (TL::QUAD && ES == 2) {
       if (nx & 3u) {
         plain_sweep();                    // quads need a row length divisible by 4
@@ -683,6 +703,15 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                     constexpr int i = I;
                     mem.template st32<TL::NT>((Off)i * NBo + selfb, xa * 2, out[i]);
                 });
+                if constexpr (ACC) {                     // both cells' stored values, decoded
+                    T sa[Q], sb[Q];
+                    static_for<0, Q>([&](auto I) {
+                        sa[I] = C::decode((uint16_t)(out[I] & 0xFFFFu), osh(I), inv_scale);
+                        sb[I] = C::decode((uint16_t)(out[I] >> 16), osh(I), inv_scale);
+                    });
+                    macro_accumulate<NORM, T>(a.acc, N, (size_t)selfrow + xa, sa);
+                    macro_accumulate<NORM, T>(a.acc, N, (size_t)selfrow + xa + 1, sb);
+                }
             }
         };
         uint32_t carry[Q] = {};
@@ -793,6 +822,15 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
                     constexpr int i = I;
                     mem.template st64<TL::NT>((Off)i * NBo + selfb, xa * 4, ob[i]);
                 });
+                if constexpr (ACC) {
+                    T sa[Q], sb[Q];
+                    static_for<0, Q>([&](auto I) {
+                        sa[I] = C::decode((uint32_t)ob[I], osh(I), inv_scale);
+                        sb[I] = C::decode((uint32_t)(ob[I] >> 32), osh(I), inv_scale);
+                    });
+                    macro_accumulate<NORM, T>(a.acc, N, (size_t)selfrow + xa, sa);
+                    macro_accumulate<NORM, T>(a.acc, N, (size_t)selfrow + xa + 1, sb);
+                }
             }
             carried = true;
         }
@@ -833,11 +871,19 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
         T u0 = (T)a.uin[z];
         if (a.gust) u0 = u0 * (T)a.gust[t];
         const T usq15 = T(1.5) * (u0 * u0);
+        T sv[Q];
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            mem.template st<false>((Off)i * NBo + selfb, 0,
-                                   C::encode(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale));
+            const typename C::B e = C::encode(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale);
+            mem.template st<false>((Off)i * NBo + selfb, 0, e);
+            if constexpr (ACC) sv[i] = C::decode(e, osh(I), inv_scale);
         });
+        if constexpr (ACC) macro_accumulate<NORM, T>(a.acc, N, (size_t)selfrow, sv);
+        (void)sv;
+    }
+    // a solid inlet-column cell: its sample is (1, 0, 0, 0) (the general path leaves x = 0 out)
+    if constexpr (ACC) {
+        if (seg == 0 && lane == 0 && (lrow[0] & SOLID_BIT)) solid_accumulate(a.acc, N, (size_t)selfrow);
     }
 }
 

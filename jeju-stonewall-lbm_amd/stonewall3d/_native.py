@@ -30,6 +30,8 @@ SIGNATURES = {
     "sw_init": (_c_int, [_p, _c_int, _p, _p, _c_int, _c_int, _c_int, _c_flt, _c_uint, _c_flt, _p, _p]),
     "sw_step": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
                          _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p]),
+    "sw_step_accumulate": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
+                                    _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p, _p]),
     "sw_step2_partials_per_step": (_c_ll, [_c_int, _c_int, _c_int]),
     "sw_step2": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
                           _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p]),

@@ -169,15 +169,19 @@ class Simulation:
         self._pending = 0
         return self
 
-    def step(self, n=1):
-        """n fused steps (lbm3d.py:358-362); with two_step, pairs of them per launch."""
+    def step(self, n=1, accumulate=False):
+        """n fused steps (lbm3d.py:358-362); with two_step, pairs of them per launch.
+        accumulate: the last of them also adds (rho, u) of its result into the time-mean
+        accumulator (k_macro + k_accum after k_step, lbm3d.py:363-370) in the same launch."""
         lib = nat.lib()
         s = _stream()
         n = int(n)
         if self.t + n > self.n_steps:
             raise RuntimeError(f"step {self.t + n - 1} beyond n_steps={self.n_steps}")
+        if accumulate and n > 0 and self.Uacc is None:
+            self.Uacc = torch.zeros(4 * self.N, dtype=torch.float32, device=self.device)
         while n > 0:
-            pair = self.two_step and n >= 2
+            pair = self.two_step and (n >= 3 or (n == 2 and not accumulate))
             layout = 2 if pair else 1
             if self.record_force and self._pending and (self._layout != layout or
                                                         self._pending + layout > self.ring):
@@ -189,6 +193,14 @@ class Simulation:
                                   self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
                                   _ptr(self.shift), _ptr(self.partials2), self.ring, s)
                 nat.check(rc, "sw_step2")
+            elif accumulate and n == 1:
+                rc = lib.sw_step_accumulate(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype,
+                                            self.links.data_ptr(), self.uin.data_ptr(), _ptr(self.gust), self.t, None,
+                                            self.nx, self.ny, self.nz, self.nu0, self.cs, self.x_sponge,
+                                            self.nu_sponge, self.ybc, self.scale, _ptr(self.shift),
+                                            _ptr(self.partials), self.ring, self.Uacc.data_ptr(), s)
+                nat.check(rc, "sw_step_accumulate")
+                self.n_acc += 1
             else:
                 rc = lib.sw_step(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
                                  self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
@@ -429,6 +441,9 @@ def _parser():
     ap.add_argument("--gust-seed", type=int, default=None, help="gusting inlet from this seed (default: steady)")
     ap.add_argument("--noise", type=float, default=0.02, help="init velocity noise amplitude (lbm3d.py:340)")
     ap.add_argument("--max-steps", type=int, default=None, help="stop after this many steps (smoke runs)")
+    ap.add_argument("--one-step", action="store_true",
+                    help="one step per kernel launch (default: two per launch where the storage allows; the "
+                         "populations are identical, the force rows equal to rounding)")
     return ap
 
 
@@ -458,8 +473,9 @@ def main(argv=None):
           f"(H/U = {su.t_unit:.0f} steps) dtype={a.dtype}", flush=True)
     wall, phi = build_wall(a.porosity, H, nx, ny, nz, X0, T, seed=a.seed)
     gust = gust_table(make_gust(a.gust_seed), n_total) if a.gust_seed is not None else None
+    two = a.dtype != "f64" and not a.one_step and Q * nx * ny * nz * nat.DTYPE_BYTES[nat.DTYPES[a.dtype]] < (1 << 32)
     sim = Simulation(wall, su.uin, nu0=su.nu0, cs=a.Cs, x_sponge=su.x_sponge, nu_sponge=0.05, ybc=a.ybc,
-                     dtype=a.dtype, gust=gust, n_steps=n_total, device=a.device, shift=a.shift)
+                     dtype=a.dtype, gust=gust, n_steps=n_total, device=a.device, shift=a.shift, two_step=two)
     y_mid, z_half = ny // 2, int(round(0.5 * H)) + 1
     ds3 = a.snap3d_ds if a.snap3d_ds > 0 else max(1, int(round(H / 13)))            # lbm3d.py:308-313
     q_x0 = max(1, X0 - int(round(0.5 * H)))
@@ -483,13 +499,30 @@ def main(argv=None):
                              snaps_q=np.array(snaps_q, np.uint8).reshape(-1, *q_dims)))
         print(f"[{a.case}] step {t_next}: state written to {ckpt}", flush=True)
 
+    def first_at_or_after(t, base, period):
+        """Smallest step k >= max(t, base) with (k - base) % period == 0."""
+        k = max(t, base)
+        return k + (-(k - base)) % period
+
+    def next_event(t):
+        """The first step >= t after which the loop samples, snapshots, checkpoints or reports."""
+        ev = [n_total - 1, first_at_or_after(t, 0, 1000), first_at_or_after(t, n_spin, 5),
+              first_at_or_after(t, n_spin, snap_every)]
+        if a.ckpt_every > 0:
+            ev.append(first_at_or_after(t + 1, 0, a.ckpt_every) - 1)
+        return min(e for e in ev if e >= t)
+
+    # The steps between two events run as one call (pairs of steps per launch where the storage
+    # allows); a sampled step adds its state to the time mean in the same launch (lbm3d.py:363-370).
     t0 = time.time()
-    for t in range(t_start, n_total):
-        sim.step(1)
+    t = t_start
+    while t < n_total:
+        e = next_event(t)
+        sampled = e >= n_spin and (e - n_spin) % 5 == 0
+        sim.step(e - t + 1, accumulate=sampled)
+        t = e
         if t >= n_spin:
             k = t - n_spin
-            if k % 5 == 0:
-                sim.accumulate()
             if k % snap_every == 0:
                 xz, xy = sim.slices(y_mid, z_half)
                 snaps_xz.append(xz)
@@ -510,6 +543,7 @@ def main(argv=None):
                 chk = f" rho[{un[0].min():.3f},{un[0].max():.3f}] |u|max/U_H={np.abs(un[1:]).max()/su.UH:.2f}"
             print(f"[{a.case}] step {t}/{n_total}  t={t/su.t_unit:.1f} H/U  {mlups:.0f} MLUPS  "
                   f"eta {el/done*(n_total-t-1)/60:.1f} min{chk}", flush=True)
+        t += 1
 
     fh = sim.force_rows()
     qref = 0.5 * su.UH ** 2

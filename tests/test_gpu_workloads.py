@@ -183,10 +183,10 @@ def _stop_at(monkeypatch, t_stop):
     from stonewall3d import lbm3d
     real = lbm3d.Simulation.step
 
-    def step(self, n=1):
+    def step(self, n=1, **kw):
         if self.t >= t_stop:
             raise _Stop()
-        return real(self, n)
+        return real(self, n, **kw)
 
     monkeypatch.setattr(lbm3d.Simulation, "step", step)
 
@@ -199,9 +199,9 @@ def test_cli_resumes_reference_format_checkpoint(gpu, tmp_path, monkeypatch):
     format (state.json with exactly the reference's signature keys, f.npy float32
     (19, nx, ny, nz), force.npy float32 (n_total, nz, 2), uacc.npy float32).  With this
     package's float64 force file removed (a directory the reference itself wrote), the
-    resumed run's outputs equal an uninterrupted run's: force-derived series exactly from
-    step 25 on (before it: the float32 rows the reference stores), means and snapshots
-    exactly."""
+    resumed run's outputs equal an uninterrupted run's: means and snapshots exactly (the
+    populations are), force-derived series to rounding (before step 25: the float32 rows the
+    reference stores)."""
     from stonewall3d import lbm3d
     straight, resumed = str(tmp_path / "a"), str(tmp_path / "b")
     lbm3d.main(["--case", "c", "--out", straight, "--ckpt-every", "0"] + CLI)
@@ -225,7 +225,10 @@ def test_cli_resumes_reference_format_checkpoint(gpu, tmp_path, monkeypatch):
     a, b = np.load(os.path.join(straight, "c.npz")), np.load(os.path.join(resumed, "c.npz"))
     assert a.files == b.files
     for k in ("Cd", "Cl", "Cm"):
-        assert np.array_equal(a[k][25:], b[k][25:]), k
+        # from step 25 on the force rows are computed alike, but the runs split their steps into
+        # launches differently (the checkpoint is an event of one of them), and one- and
+        # two-step launches sum the force partials in different fixed orders
+        assert np.allclose(a[k][25:], b[k][25:], rtol=1e-6, atol=1e-9), k
         assert np.allclose(a[k][:25], b[k][:25], rtol=1e-5, atol=1e-7), k
     assert np.allclose(a["fx_profile"], b["fx_profile"], rtol=1e-5, atol=1e-8)
     for k in ("mean_xz_mid", "mean_xy_half", "mean3d_ds2", "snaps_xz", "snaps_xy", "snaps_q", "wall_ds2"):
