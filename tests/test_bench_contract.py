@@ -32,3 +32,21 @@ def test_traffic_summaries(dtype, bpu):
     # measured traffic lies within a few per cent of the algorithmic bytes
     assert 0.95 < t["bytes_per_launch"] / t["algorithmic_bytes"] < 1.10
     assert all("bench.py" in c for c in t["commands"])
+
+
+def test_two_step_traffic_summary():
+    """k_step2 advances two steps per launch: its HBM traffic lies between the bytes of one
+    pass over the populations (152 B per cell) and those of two single steps."""
+    p = os.path.join(ROOT, "profiles", "k_step2_porous_wall_640x320x160_f32_traffic.json")
+    with open(p) as f:
+        t = json.load(f)
+    cells = 640 * 320 * 160
+    assert t["steps_per_launch"] == 2
+    assert t["algorithmic_bytes"] == 2 * 152 * cells
+    assert t["bytes_per_launch"] == pytest.approx(t["fetch_bytes"] + t["write_bytes"], abs=2)
+    assert t["fetch_bytes"] == pytest.approx(t["fetch_size_kib_raw"] * 1024 * t["fetch_scale"], rel=1e-4)
+    assert t["write_bytes"] == pytest.approx(t["write_size_kib_raw"] * 1024 * t["write_scale"], rel=1e-4)
+    assert 152 * cells < t["bytes_per_launch"] < t["algorithmic_bytes"]
+    # every population is written exactly once per launch
+    assert t["write_bytes"] == pytest.approx(76 * cells, rel=0.02)
+    assert all("bench.py" in c for c in t["commands"])
