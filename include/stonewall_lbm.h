@@ -129,9 +129,23 @@ long long sw_step2_partials_per_step(int nx, int ny, int nz);
 int sw_step2(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
              int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
              int ybc, float scale, const float* shift, double* partials, int ring, void* stream);
+/* One step with sw_step2's tiles and force-partials layout: the odd step of a run of
+ * sw_step2 pairs (no reference counterpart: lbm3d.py:359-361 launches k_step once per step),
+ * so the whole run folds into force rows through one sw_force_reduce2 ring.  Arguments and
+ * checks as sw_step2; writes slot t % ring; populations bit-identical to sw_step. */
+int sw_step2_one(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
+                 int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge,
+                 double nu_sponge, int ybc, float scale, const float* shift, double* partials, int ring, void* stream);
 /* force[t][z][0..1] from the sw_step2 partials ring, as sw_force_reduce does for sw_step's. */
 int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int nx, int ny, int nz,
                      double* force, void* stream);
+/* Tile rows of the two-step kernels (no reference counterpart): 0 (default) picks 15 or 16 per
+ * lattice to fill the compute units in the fewest rounds; 15 or 16 fixes it for the process.
+ * The rows set the partials layout (sw_step2_partials_per_step, sw_force_reduce2): keep one
+ * setting for the life of a partials ring.  Populations do not depend on it; force rows
+ * change in rounding only.  The two-step batch uses 16. */
+int sw_set_step2_rows(int rows);
+int sw_get_step2_rows(void);
 
 /* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
  * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept
@@ -237,7 +251,7 @@ typedef struct sw_instance {
     int ring, x_sponge, ybc, dtype;
     int t, t_end, parity;     /* next step, end step (exclusive), buffer of step t */
     int item0;                /* first work item; written by sw_batch_upload */
-    int reserved;
+    int tile0;                /* first two-step work item (tile); written by sw_batch_upload */
 } sw_instance;
 
 /* Bytes of a device table holding up to max_instances lattices. */
@@ -257,6 +271,26 @@ int sw_batch_advance(void* table, int max_instances, int K, void* stream);
  * replay with sw_graph_launch on any stream, free with sw_graph_destroy. */
 int sw_batch_graph_create(void* table, int max_instances, int dtype, int K, int blocks, int variant, int big_offsets,
                           void** graph);
+/* Two-step form of the batch (no reference counterpart, as above): every instance FP32, FP16
+ * or BF16 (one dtype per table) with population arrays below 4 GiB.  Partials ring per
+ * instance: ring * sw_batch_partials_per_step(nx, ny, nz) doubles (enough for either form).
+ * sw_batch_step2(p): steps t_i + 2p and t_i + 2p + 1 of every instance with two or more left
+ * at t_i + 2p, persistent `blocks` workgroups (one per compute unit fits) taking 64 x 14-cell
+ * tiles from a counter in the table header; the buffers swap once per launch that touches
+ * an instance.  sw_batch_force_reduce2: force rows of steps t_i .. t_i + K - 1 (ring >= K).
+ * sw_batch_advance2: t_i += K (K even, clamped to t_end) and the parity of the launches.
+ * sw_batch_graph_create2: K / 2 pair launches + the tail launch + force reduction + advance
+ * as one hipGraph.
+ * Populations are bit-identical to single steps; force rows to the single-lattice run with
+ * sw_set_step2_rows(16) (pairs by sw_step2, an odd last step by sw_step2_one). */
+long long sw_batch_partials_per_step(int nx, int ny, int nz);
+int sw_batch_step2(void* table, int dtype, int p, int blocks, void* stream);
+/* After the K / 2 pair launches of a chunk: the odd last step of every instance with fewer
+ * than K (and an odd number of) steps left at the chunk's start. */
+int sw_batch_step2_tail(void* table, int dtype, int K, int blocks, void* stream);
+int sw_batch_force_reduce2(const void* table, int max_instances, int K, void* stream);
+int sw_batch_advance2(void* table, int max_instances, int K, void* stream);
+int sw_batch_graph_create2(void* table, int max_instances, int dtype, int K, int blocks, void** graph);
 int sw_graph_launch(void* graph, void* stream);
 int sw_graph_destroy(void* graph);
 

@@ -11,7 +11,7 @@
 
 namespace sw {
 
-constexpr int SW_VERSION = 110;   // 1.1.0
+constexpr int SW_VERSION = 120;   // 1.2.0 (two-step batch; sw_instance.tile0; table holds per-lattice arguments)
 // library error codes (SW_ERR_*) come from stonewall_lbm.h; hipError_t values (< 1000)
 // are passed through unchanged
 

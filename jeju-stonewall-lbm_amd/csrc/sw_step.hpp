@@ -90,8 +90,8 @@ __device__ __forceinline__ Moments<T> moments_norm(const T (&fl)[Q]) {
     return m;
 }
 
-template <bool NORM, class T, class Store>
-__device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, const StepArgs& a, T smag, Store&& st) {
+template <bool NORM, class T, class A, class Store>
+__device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, A& a, T smag, Store&& st) {
     T rho, mx, my, mz, pxx, pyy, pzz, pxy, pxz, pyz;
     if constexpr (NORM) {
         const Moments<T> m = moments_norm<T, true>(fl);
@@ -300,8 +300,10 @@ constexpr int NUM_TILE_VARIANTS = 14;
 // BUF selects raw-buffer addressing (all byte offsets of a population array < 4 GiB).
 // ACC: also add (rho, u) of every cell's output state into a.acc (k_macro + k_accum on the
 // step's result, lbm3d.py:363-370), without re-reading the populations.
-template <class S, class TL, bool BUF, bool ACC = false>
-__device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t, int lane, int seg,
+// A: StepArgs, or a StepArgs in the constant address space (the batched kernels' per-lattice
+// arguments, whose loads the compiler may repeat instead of keeping them in registers).
+template <class S, class TL, bool BUF, bool ACC = false, class A>
+__device__ __forceinline__ void step_row(A& a, int y, int z, int t, int lane, int seg,
                                          double& fx_acc, double& fz_acc) {
     using C = Codec<S>;
     using T = typename C::T;
@@ -889,8 +891,8 @@ __device__ __forceinline__ void step_row(const StepArgs& a, int y, int z, int t,
 
 // Writes this workgroup's FP64 force pair (sum over its WPB wavefronts in wavefront
 // order) to partials[slot][z][yb].
-template <int WPB>
-__device__ __forceinline__ void block_force(const StepArgs& a, double fx, double fz, int z, int yb, int t) {
+template <int WPB, class A>
+__device__ __forceinline__ void block_force(A& a, double fx, double fz, int z, int yb, int t) {
     __shared__ double red[WPB][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     fx = wave_sum(fx);

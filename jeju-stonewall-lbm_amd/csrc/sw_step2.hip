@@ -1,504 +1,46 @@
-// Two fused steps per pass (temporal blocking): f(t) -> f(t+1) -> f(t+2) in one launch, the
-// intermediate state kept on chip, so each pass reads and writes the populations once for
-// two updates.  Restates two applications of stonewall3d/lbm3d.py:57-155 (k_step) with the
-// arithmetic of sw_step.hpp, so the populations are bit-identical to two single steps.
-//
-// Work decomposition.  A workgroup owns a tile of 64 cells in x (aligned to 64: whole
-// 64-cell chunks, so every load and store of a row is one aligned segment) and TY rows in
-// y, of which rows 1..TY-2 are its output rows and rows 0 and TY-1 halo rows, and sweeps
-// the tile through z.  Wavefront w owns row w; lane L the cell x0 + L.  Iteration k:
-//   phase 1  every wavefront computes the first step at plane k for its row (f1(k)),
-//            keeps the populations that stay in its row and column in registers, writes
-//            the ones that cross rows (c_y != 0) to LDS, and issues the loads of f1(k+1);
-//   barrier
-//   phase 2  the output rows compute the second step at plane k-1 (f2(k-1)) from f1 of
-//            planes k-2, k-1, k (registers, LDS, x-neighbours by DPP lane shifts) and store
-//            it; the two halo-row wavefronts compute f1(k+1) of the cells just outside the
-//            tile in x (x0-1 and x0+64, every row), whose c_x != 0 populations the tile's
-//            edge lanes pull in the next iteration, and wavefront 0 writes the force
-//            partials of the previous plane.
-// One barrier per plane: LDS rings are deep enough (4 slots for c_z = +1, 3 for the rest)
-// that a wavefront writing plane k+1 never overwrites a slot another wavefront still reads.
-//
-// The force of step t (f1) and t+1 (f2) is summed per tile and plane (lane FP64 sums, a
-// fixed butterfly per wavefront, wavefronts in order) into partials[slot][z][tile][2]; the
-// rows of force come from sw_force_reduce2.  The summation order differs from the single
-// step's (per row-group partials), so force rows agree with it to rounding, not bit for bit.
-#include "sw_step.hpp"
+// Single-lattice entry points of the two-step kernel (the tile function: sw_step2.hpp).
+#include "sw_step2.hpp"
 #include "sw_api_util.hpp"
 
 namespace sw {
 
-// LDS of one workgroup.  Row arrays have 66 columns: column L + 1 holds lane L's cell, column
-// 0 the cell x0-1 and column 65 the cell x0+64 (written by the halo computation).
-template <int TY>
-struct Step2Lds {
-    float A[4][2][TY][66];   // c_z = +1, c_y != 0: populations 15, 18 (read two planes later)
-    float B[3][6][TY][66];   // c_z = 0,  c_y != 0: populations 3, 4, 7, 8, 9, 10
-    float C[3][2][TY][66];   // c_z = -1, c_y != 0: populations 16, 17
-    float H[4][6][TY];       // halo cells, in-row populations with c_x != 0:
-                             // 1, 11, 13 of x0-1 and 2, 12, 14 of x0+64
-    double red[3][TY][4];    // per-wavefront force sums (f1 Fx, Fz; f2 Fx, Fz) of three iterations
-};
-
-__host__ __device__ constexpr int s2_gidx(int i) {      // index within the LDS group of a c_y != 0 population
-    return i == 15 ? 0 : i == 18 ? 1 : i == 16 ? 0 : i == 17 ? 1 : i == 3 ? 0 : i == 4 ? 1 : i == 7 ? 2
-         : i == 8 ? 3 : i == 9 ? 4 : i == 10 ? 5 : -1;
-}
-__host__ __device__ constexpr int s2_hidx(int i) {      // index of an in-row c_x != 0 population in H
-    return i == 1 ? 0 : i == 11 ? 1 : i == 13 ? 2 : i == 2 ? 3 : i == 12 ? 4 : i == 14 ? 5 : -1;
-}
-
-template <int TY>
-__device__ __forceinline__ float& lds_row(Step2Lds<TY>& L, int j, int plane, int gi, int r, int col) {
-    if (CZ[j] > 0) return L.A[plane & 3][gi][r][col];
-    if (CZ[j] == 0) return L.B[plane % 3][gi][r][col];
-    return L.C[plane % 3][gi][r][col];
+template <class S, int TY>
+__global__ __launch_bounds__(64 * TY, 1) void k_step2(StepArgs a, int ntx, int ntiles) {
+    // (Numbering the tiles so that each XCD, with its own L2, gets runs of neighbouring tiles
+    // measured no faster: the workgroup order is plain.)
+    __shared__ Step2Lds<TY> L;
+    step2_tile<S, TY, false>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, (int)blockIdx.x, L);
 }
 
 template <class S, int TY>
-__global__ __launch_bounds__(64 * TY, 1) void k_step2(StepArgs a, int ntx, int ntiles) {
-    using Cd = Codec<S>;
-    using T = typename Cd::T;
-    using B = typename Cd::B;
-    constexpr int OY = TY - 2;
-    constexpr bool NORM = sizeof(S) == 2;
-    constexpr uint32_t ES = sizeof(S);
+__global__ __launch_bounds__(64 * TY, 1) void k_step2_one(StepArgs a, int ntx, int ntiles) {
     __shared__ Step2Lds<TY> L;
-
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // (Numbering the tiles so that each XCD, with its own L2, gets runs of neighbouring tiles
-    // measured no faster: the workgroup order is plain.)
-    const int tile = (int)blockIdx.x;
-    const int tx = tile % ntx, ty = tile / ntx;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const uint32_t N = (uint32_t)nx * ny * nz;
-    const uint32_t NB = N * ES;                       // bytes per population
-    const uint32_t PB = (uint32_t)nx * ny * ES;       // bytes per z plane of a population
-    const int t0 = a.t + (a.t_dev ? *a.t_dev : 0);
-    const bool periodic = a.ybc == Y_PERIODIC;
-    const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.f_in), rout = make_rsrc(a.f_out);
-    const T scale = (T)a.scale;
-    const T inv_scale = (T)a.inv_scale;
-    const int kscale = (int)((__builtin_bit_cast(uint32_t, a.scale) >> 23) & 0xFFu) - 127;
-    const T cs = (T)a.cs;
-    const T smag = ((T(18) * (T)sqrt(T(2))) * cs) * cs;
-    const uint32_t cw = chunk_words(nx);
-    const uint32_t* cflags = a.links + N;
-
-    // ---- this wavefront's row
-    const int yg = ty * OY - 1 + w;                   // may be -1 or >= ny
-    bool need = yg <= ny;                             // rows past the tile's last halo row are idle
-    int y;
-    if (periodic) {
-        y = yg < 0 ? yg + ny : (yg >= ny ? yg - ny : yg);
-    } else {
-        need = need && yg >= 0 && yg < ny;            // free-slip: no row outside the lattice
-        y = yg < 0 ? 0 : (yg >= ny ? ny - 1 : yg);
-    }
-    const bool out_row = w >= 1 && w <= OY && yg < ny;
-    // source rows of c_y = +1 / -1 links after the side rule; mlo / mhi: mirrored (free-slip)
-    int ym = y - 1, yp = y + 1;
-    bool mlo = false, mhi = false;
-    if (periodic) {
-        ym = ym < 0 ? ym + ny : ym;
-        yp = yp >= ny ? yp - ny : yp;
-    } else {
-        if (ym < 0) { ym = y; mlo = true; }
-        if (yp >= ny) { yp = y; mhi = true; }
-    }
-    const int x0 = tx * 64;
-    const int x = x0 + lane;
-    const int xl = x < nx ? x : nx - 1;              // lanes past the row end work on in-row addresses
-    const uint32_t xo = (uint32_t)xl * ES;
-    const uint32_t xpo = (uint32_t)(xl + 1 < nx ? xl + 1 : nx - 1) * ES;
-    const bool outlet = x == nx - 1;
-
-    // offsets of the 16-bit storage: sigma_c(z) of the c_x class c at height z
-    auto sig = [&](int z, int c) -> T { return a.shift ? (T)a.shift[z * 4 + c] : T(1); };
-
-    // byte offset of link I's source row at plane k (lbm3d.py:77-86 with the side rules)
-    auto src_off = [&](auto I, int k, bool lo, bool hi, int yrm, int yr0, int yrp) -> uint32_t {
-        constexpr int i = I;
-        int j = i;
-        if constexpr (CY[i] > 0) { if (lo) j = MIRY[i]; }
-        if constexpr (CY[i] < 0) { if (hi) j = MIRY[i]; }
-        int zs = k - CZ[i];
-        if constexpr (CZ[i] < 0) {
-            if (zs >= nz) { zs = k; j = (CY[i] > 0 && lo) || (CY[i] < 0 && hi) ? MIRZ[MIRY[i]] : MIRZ[i]; }
-        }
-        const int ys = CY[i] > 0 ? yrm : (CY[i] < 0 ? yrp : yr0);
-        return (uint32_t)j * NB + (uint32_t)zs * PB + (uint32_t)ys * (uint32_t)nx * ES;
-    };
-    // source-row offset of link I at plane k for this wavefront's row (side rules applied;
-    // the top plane's c_z = -1 links mirror onto their own plane, src_off)
-    const uint32_t ybm = (uint32_t)ym * (uint32_t)nx * ES, yb0 = (uint32_t)y * (uint32_t)nx * ES,
-                   ybp = (uint32_t)yp * (uint32_t)nx * ES;
-    auto row_off = [&](auto I, int k) -> uint32_t {
-        constexpr int i = I;
-        if constexpr (CZ[i] < 0) {
-            if (k + 1 >= nz) return src_off(I, k, mlo, mhi, ym, y, yp);
-        }
-        uint32_t j = i;
-        if constexpr (CY[i] > 0) j = mlo ? (uint32_t)MIRY[i] : (uint32_t)i;
-        if constexpr (CY[i] < 0) j = mhi ? (uint32_t)MIRY[i] : (uint32_t)i;
-        const uint32_t yb = CY[i] > 0 ? ybm : (CY[i] < 0 ? ybp : yb0);
-        return j * NB + (uint32_t)(k - CZ[i]) * PB + yb;
-    };
-    // decode offset of link I's source at plane k (the source height; the top row's c_z = -1
-    // links mirror onto its own height)
-    auto dsh = [&](auto I, int k) -> T {
-        constexpr int i = I;
-        if constexpr (NORM) {
-            const int zs = CZ[i] > 0 ? k - 1 : (CZ[i] < 0 ? (k + 1 < nz ? k + 1 : k) : k);
-            return sig(zs, xclass(CX[i]));
-        } else {
-            return wgt<T>(i);
-        }
-    };
-    // value a following step reads back from the storage of v written at height k: identity
-    // for FP32; for the 16-bit forms the encode with the offsets of height k, then the decode
-    auto stored = [&](auto I, T v, int k) -> T {
-        constexpr int i = I;
-        if constexpr (NORM) {
-            const T s = sig(k, xclass(CX[i]));
-            return Cd::decode(Cd::encode(v, (T)neg_scaled((float)s, kscale), scale), s, inv_scale);
-        } else {
-            return v;
-        }
-    };
-    auto chunk_fast = [&](int row, int k, int c) -> bool {
-        return !((cflags[((uint32_t)k * ny + row) * cw + (c >> 5)] >> (c & 31)) & 1u);
-    };
-    auto inlet_u0 = [&](int k, int t) -> T {
-        T u0 = (T)a.uin[k];
-        if (a.gust) u0 = u0 * (T)a.gust[t];
-        return u0;
-    };
-    auto wall_force = [&](auto I, T f, T& fxw, T& fzw) {
-        constexpr int i = I;
-        constexpr T wi = wgt<T>(i);
-        const T fr2 = NORM ? (T(2) * wi) * (f - T(1)) : T(2) * (f - wi);
-        if constexpr (CX[i] > 0) fxw = fxw - fr2;
-        if constexpr (CX[i] < 0) fxw = fxw + fr2;
-        if constexpr (CZ[i] > 0) fzw = fzw - fr2;
-        if constexpr (CZ[i] < 0) fzw = fzw + fr2;
-    };
-    auto is_wall = [&](auto I, uint32_t mm) -> bool {
-        constexpr int i = I;
-        bool wall = (mm >> i) & 1u;
-        if constexpr (GROUND_SLOT[i] >= 0) wall = wall && !((mm >> (GROUND_BIT0 + GROUND_SLOT[i])) & 1u);
-        return wall;
-    };
-
-    // ---- first step, fast path loads of plane k for this wavefront's row
-    auto f1_load = [&](int k, T (&fl)[Q]) {
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            const uint32_t so = row_off(I, k) - (CX[i] > 0 ? ES : 0u);
-            fl[i] = Cd::decode(buf_load<B>(rin, CX[i] < 0 ? xpo : xo, so), dsh(I, k), inv_scale);
-        });
-    };
-    // ---- first step near solids: the streamed values are loaded as on the fast path; on
-    // links whose source is solid (mask bit set) the cell's own opposite population replaces
-    // them (half-way bounce-back, lbm3d.py:87-98), loaded only where a lane of the wavefront
-    // needs it, all issued before the first is used; then the wall force (lbm3d.py:89-101).
-    // self: byte offset of the cell in a population array; k: its plane.
-    auto bounce_fixups = [&](int k, uint32_t self, uint32_t mm, T (&fl)[Q], T& fxw, T& fzw) {
-        B own[Q];
-        static_for<1, Q>([&](auto I) {
-            constexpr int i = I;
-            if (__builtin_amdgcn_ballot_w64((mm >> i) & 1u) != 0) own[i] = buf_load<B>(rin, self, (uint32_t)OPP[i] * NB);
-        });
-        static_for<1, Q>([&](auto I) {
-            constexpr int i = I;
-            if ((mm >> i) & 1u) {
-                T off;
-                if constexpr (NORM) off = sig(k, xclass(-CX[i]));
-                else off = wgt<T>(OPP[i]);
-                fl[i] = Cd::decode(own[i], off, inv_scale);
-            }
-            if (is_wall(I, mm)) wall_force(I, fl[i], fxw, fzw);
-        });
-    };
-
-    // ---- halo cells (x0-1, x0+64) of every row at plane kh: f1 into H and the row arrays'
-    // edge columns.  Lane l of wavefront 0 (TY-1) takes row l/2 (TY/2 + l/2), side l & 1:
-    // the streamed values of plane kh, the bounce-back, the collision and the values the
-    // tile's edge lanes pull.  It runs in phase 2, where the halo-row wavefronts have no second
-    // step to compute.  (Issuing its loads in phase 1 instead holds 19 more registers across
-    // the barrier and measured 5 % slower.)
-    const bool halo_wave = (w == 0 || w == TY - 1);
-    // (the per-lane set-up is recomputed on each call rather than held in registers across
-    // the sweep, where the row wavefronts need them)
-    auto halo_f1 = [&](int kh) {
-        const int hr = ((w == 0) ? 0 : TY / 2) + (lane >> 1);
-        const int hside = lane & 1;
-        const int hx = hside ? x0 + 64 : x0 - 1;
-        const int hyg = ty * OY - 1 + hr;
-        if (!(hr < ((w == 0) ? TY / 2 : TY) && (hside ? hx < nx : tx > 0) && hyg <= ny)) return;
-        int hy;
-        if (periodic) {
-            hy = hyg < 0 ? hyg + ny : (hyg >= ny ? hyg - ny : hyg);
-        } else {
-            if (hyg < 0 || hyg >= ny) return;
-            hy = hyg;
-        }
-        int hym = hy - 1, hyp = hy + 1;
-        bool hlo = false, hhi = false;
-        if (periodic) {
-            hym = hym < 0 ? hym + ny : hym;
-            hyp = hyp >= ny ? hyp - ny : hyp;
-        } else {
-            if (hym < 0) { hym = hy; hlo = true; }
-            if (hyp >= ny) { hyp = hy; hhi = true; }
-        }
-        const uint32_t hxo = (uint32_t)hx * ES, hxm = (uint32_t)(hx > 0 ? hx - 1 : 0) * ES,
-                       hxp = (uint32_t)(hx + 1 < nx ? hx + 1 : nx - 1) * ES;
-        const uint32_t hmask = a.links[((uint32_t)kh * ny + hy) * nx + hx];
-        B hraw[Q];
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            hraw[i] = buf_load<B>(rin, src_off(I, kh, hlo, hhi, hym, hy, hyp) + (CX[i] < 0 ? hxp : (CX[i] > 0 ? hxm : hxo)),
-                                  0u);
-        });
-        if (hmask & SOLID_BIT) return;                   // a solid cell's populations are never read
-        T fl[Q];
-        static_for<0, Q>([&](auto I) { fl[I] = Cd::decode(hraw[I], dsh(I, kh), inv_scale); });
-        T fxw = T(0), fzw = T(0);
-        bounce_fixups(kh, (((uint32_t)kh * ny + hy) * nx + hx) * ES, hmask, fl, fxw, fzw);
-        collide<NORM, T>(fl, (uint32_t)hx, a, smag, [&](auto I, T val) {
-            constexpr int i = I;
-            if constexpr (CX[i] != 0) {
-                if ((CX[i] > 0) == (hside == 0)) {       // x0-1 feeds c_x = +1, x0+64 feeds c_x = -1
-                    const T p = stored(I, val, kh);
-                    if constexpr (CY[i] == 0) L.H[kh & 3][s2_hidx(i)][hr] = (float)p;
-                    else L.B[kh % 3][s2_gidx(i)][hr][hside ? 65 : 0] = (float)p;
-                }
-            }
-        });
-    };
-
-    // one wavefront's FP64 sum of its lanes' per-cell FP32 force into red[slot][w][c0..c0+1]
-    // (a fixed butterfly; skipped when no lane has a wall link)
-    auto wave_force = [&](float fx, float fz, int slot, int c0) {
-        double sx = 0.0, sz = 0.0;
-        if (__builtin_amdgcn_ballot_w64(fx != 0.0f || fz != 0.0f) != 0) {
-            sx = wave_sum((double)fx);
-            sz = wave_sum((double)fz);
-        }
-        if (lane == 0) { L.red[slot][w][c0] = sx; L.red[slot][w][c0 + 1] = sz; }
-    };
-    // ---- force partials of one plane: wavefront sums in order
-    const bool force = a.partials != nullptr;
-    const int P = ntiles;
-    // iteration kf's sums sit in red[kf % 3]: phase 1 writes them at iteration kf, wavefront 0
-    // reads them in phase 2 of iteration kf + 1, and they are rewritten only at iteration
-    // kf + 3, after a barrier that wavefront 0 reaches once it has read them
-    auto flush = [&](int slot_plane) {                   // plane whose f1 force sits in red[slot_plane % 3]
-        if (!force || lane != 0) return;
-        const int s = slot_plane % 3;
-        double f1x = 0.0, f1z = 0.0, f2x = 0.0, f2z = 0.0;
-        for (int r = 1; r <= OY; r++) {
-            f1x += L.red[s][r][0]; f1z += L.red[s][r][1];
-            f2x += L.red[s][r][2]; f2z += L.red[s][r][3];
-        }
-        const int k1 = slot_plane, k2 = slot_plane - 1;
-        if (k1 >= 1 && k1 < nz) {
-            double* p = a.partials + (((size_t)(t0 % a.ring) * nz + k1) * P + tile) * 2;
-            p[0] = f1x; p[1] = f1z;
-        }
-        if (k2 >= 1 && k2 < nz) {
-            double* p = a.partials + (((size_t)((t0 + 1) % a.ring) * nz + k2) * P + tile) * 2;
-            p[0] = f2x; p[1] = f2z;
-        }
-    };
-
-    // ---- prologue: the ground plane's partials are zero; halo and loads of plane 1
-    if (force && w == 0 && lane == 0) {
-        for (int s = 0; s < 2; s++) {
-            double* p = a.partials + (((size_t)((t0 + s) % a.ring) * nz + 0) * P + tile) * 2;
-            p[0] = 0.0; p[1] = 0.0;
-        }
-    }
-    if (halo_wave && nz > 1) halo_f1(1);
-
-    T h0[3] = {}, hp1[3] = {}, hp2[3] = {}, hm1[3] = {};   // own-column f1 populations kept in registers
-    T c6 = T(0), c12 = T(0), c13 = T(0);                     // f1(k) of populations 6, 12, 13
-    uint32_t m_prev = 0;                                     // link mask of this lane's cell at plane k-1
-
-#pragma clang loop unroll(disable)
-    for (int k = 1; k <= nz; k++) {
-        // ================= phase 1: f1 at plane k
-        T pk[Q];                                  // what the second step reads of f1(k)
-        uint32_t m_cur = 0;
-        if (need && k < nz) {
-            T fl[Q];
-            T fxw = T(0), fzw = T(0);
-            bool live = true;
-            const bool fast = chunk_fast(y, k, tx);
-            if (!fast) m_cur = a.links[((uint32_t)k * ny + y) * nx + xl];
-            f1_load(k, fl);
-            if (!fast) {
-                live = !(m_cur & SOLID_BIT);
-                bounce_fixups(k, (((uint32_t)k * ny + y) * nx + xl) * ES, m_cur, fl, fxw, fzw);
-            }
-            T v[Q];
-            collide<NORM, T>(fl, (uint32_t)xl, a, smag, [&](auto I, T val) { v[I] = val; });
-            if (x == 0) {                                                    // inlet (lbm3d.py:67-71)
-                const T u0 = inlet_u0(k, t0);
-                const T usq15 = T(1.5) * (u0 * u0);
-                static_for<0, Q>([&](auto I) { v[I] = inlet_value<NORM, I, T>(u0, usq15); });
-            }
-            static_for<0, Q>([&](auto I) { pk[I] = stored(I, v[I], k); });
-            if (force) wave_force(out_row && x < nx && x != 0 && live ? (float)fxw : 0.0f,
-                                  out_row && x < nx && x != 0 && live ? (float)fzw : 0.0f, k % 3, 0);
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = I;
-                if constexpr (CY[i] != 0) lds_row(L, i, k, s2_gidx(i), w, lane + 1) = (float)pk[i];
-            });
-        } else {
-            static_for<0, Q>([&](auto I) { pk[I] = T(0); });
-            if (force && lane == 0) { L.red[k % 3][w][0] = 0.0; L.red[k % 3][w][1] = 0.0; }
-        }
-        c6 = pk[6]; c12 = pk[12]; c13 = pk[13];
-        __syncthreads();
-
-        // ================= phase 2: f2 at plane kk = k-1 (output rows); halo f1 at k+1 (halo rows)
-        const int kk = k - 1;
-        float fbx = 0.0f, fbz = 0.0f;
-        if (out_row && kk >= 1) {
-            const bool top = kk == nz - 1;
-            T g[Q];
-            auto shr = [&](T v, T edge) { return __uint_as_float(wave_shr1(__float_as_uint(v), __float_as_uint(edge))); };
-            auto shl = [&](T v, T edge) {
-                const T s = __uint_as_float(wave_shl1(__float_as_uint(v), __float_as_uint(edge)));
-                return outlet ? v : s;                                       // outlet: the cell pulls itself
-            };
-            g[0] = h0[0];
-            g[1] = shr(h0[1], L.H[kk & 3][0][w]);
-            g[2] = shl(h0[2], L.H[kk & 3][3][w]);
-            g[5] = hp2[0];
-            g[11] = shr(hp2[1], L.H[(kk - 1) & 3][1][w]);
-            g[14] = shl(hp2[2], L.H[(kk - 1) & 3][5][w]);
-            if (!top) {
-                g[6] = c6;
-                g[12] = shl(c12, L.H[k & 3][4][w]);
-                g[13] = shr(c13, L.H[k & 3][2][w]);
-            } else {                                                         // top: mirror onto plane kk
-                g[6] = hp1[0];
-                g[12] = shl(hp1[2], L.H[kk & 3][5][w]);
-                g[13] = shr(hp1[1], L.H[kk & 3][1][w]);
-            }
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = I;
-                if constexpr (CY[i] != 0) {
-                    const bool ym_ = CY[i] > 0 ? mlo : mhi;                 // free-slip mirror at this row
-                    const int r = ym_ ? w : w - CY[i];
-                    const int col = (CX[i] < 0 && outlet) ? lane + 1 : lane + 1 - CX[i];
-                    if constexpr (CZ[i] < 0) {
-                        if (top) {                                          // onto plane kk, population MIRZ[j]
-                            constexpr int jz = MIRZ[i], jyz = MIRZ[MIRY[i]];
-                            g[i] = lds_row(L, jz, kk, ym_ ? s2_gidx(jyz) : s2_gidx(jz), r, col);
-                        } else {
-                            g[i] = lds_row(L, i, k, ym_ ? s2_gidx(MIRY[i]) : s2_gidx(i), r, col);
-                        }
-                    } else {
-                        g[i] = lds_row(L, i, kk - CZ[i], ym_ ? s2_gidx(MIRY[i]) : s2_gidx(i), r, col);
-                    }
-                }
-            });
-            // the cell's own f1 populations at plane kk (bounce-back sources)
-            auto own = [&](auto J) -> T {
-                constexpr int j = J;
-                if constexpr (j == 0 || j == 1 || j == 2) return h0[j];
-                else if constexpr (j == 5) return hp1[0];
-                else if constexpr (j == 11) return hp1[1];
-                else if constexpr (j == 14) return hp1[2];
-                else if constexpr (j == 6) return hm1[0];
-                else if constexpr (j == 12) return hm1[1];
-                else if constexpr (j == 13) return hm1[2];
-                else return lds_row(L, j, kk, s2_gidx(j), w, lane + 1);
-            };
-            bool live = true;
-            T fxw = T(0), fzw = T(0);
-            if (!chunk_fast(y, kk, tx)) {
-                const uint32_t mm = m_prev;
-                live = !(mm & SOLID_BIT);
-                static_for<1, Q>([&](auto I) {
-                    constexpr int i = I;
-                    if ((mm >> i) & 1u) g[i] = own(std::integral_constant<int, OPP[i]>{});
-                    if (is_wall(I, mm)) wall_force(I, g[i], fxw, fzw);
-                });
-            }
-            const T en0 = (T)neg_scaled((float)sig(kk, 0), kscale), en1 = (T)neg_scaled((float)sig(kk, 1), kscale),
-                    en2 = (T)neg_scaled((float)sig(kk, 2), kscale);
-            auto esh = [&](auto I) -> T {
-                constexpr int i = I;
-                if constexpr (NORM) return xclass(CX[i]) == 0 ? en0 : (xclass(CX[i]) == 1 ? en1 : en2);
-                else return wgt<T>(i);
-            };
-            const bool st_ok = x < nx && live;
-            const uint32_t rowb = ((uint32_t)kk * ny + (uint32_t)y) * (uint32_t)nx * ES;
-            if (x == 0) {                                                    // inlet
-                const T u0 = inlet_u0(kk, t0 + 1);
-                const T usq15 = T(1.5) * (u0 * u0);
-                static_for<0, Q>([&](auto I) {
-                    constexpr int i = I;
-                    if (st_ok) buf_store<B, 2>(Cd::encode(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale), rout, xo,
-                                               (uint32_t)i * NB + rowb);
-                });
-            } else {
-                collide<NORM, T>(g, (uint32_t)xl, a, smag, [&](auto I, T val) {
-                    constexpr int i = I;
-                    if (st_ok) buf_store<B, 2>(Cd::encode(val, esh(I), scale), rout, xo, (uint32_t)i * NB + rowb);
-                });
-                if (st_ok) { fbx = (float)fxw; fbz = (float)fzw; }
-            }
-        }
-        if (halo_wave && k + 1 < nz) halo_f1(k + 1);
-        if (w == 0 && k >= 2) flush(k - 1);
-        // per-wavefront force sums of this iteration
-        if (force) wave_force(fbx, fbz, k % 3, 2);
-        // rotate the registers: plane k becomes k-1
-        hp2[0] = hp1[0]; hp2[1] = hp1[1]; hp2[2] = hp1[2];
-        hp1[0] = pk[5]; hp1[1] = pk[11]; hp1[2] = pk[14];
-        h0[0] = pk[0]; h0[1] = pk[1]; h0[2] = pk[2];
-        hm1[0] = c6; hm1[1] = c12; hm1[2] = c13;
-        m_prev = m_cur;
-    }
-    __syncthreads();
-    if (w == 0) flush(nz);
+    step2_tile<S, TY, true>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, (int)blockIdx.x, L);
 }
 
-// Tile rows per workgroup: the candidate that fills the compute units in the fewest rounds
-// of workgroups, each round weighted by its rows per output row.
-template <class S>
+// Tile rows per workgroup (15 or 16) for a lattice: the setting of sw_set_step2_rows, or by
+// default the candidate that fills the compute units in the fewest rounds of workgroups,
+// each round weighted by its rows per output row.
+static int g_step2_rows = 0;
+
+static int step2_rows(int nx, int ny, int cus) {
+    if (g_step2_rows) return g_step2_rows;
+    auto cost = [&](int ty) { return ((step2_tiles_for(nx, ny, ty) + cus - 1) / cus) * ty; };
+    return cost(15) < cost(16) ? 15 : 16;
+}
+
+template <class S, bool SINGLE>
 static void launch_step2(const StepArgs& a, int cus, hipStream_t s) {
     const int ntx = (a.nx + 63) / 64;
-    auto cost = [&](int ty) {                      // rounds of workgroups x rows each workgroup sweeps
-        const int tiles = ntx * ((a.ny + ty - 3) / (ty - 2));
-        return ((tiles + cus - 1) / cus) * ty;
-    };
-    if (cost(15) < cost(16)) {
-        const int tiles = ntx * ((a.ny + 12) / 13);
-        hipLaunchKernelGGL((k_step2<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles);
+    if (step2_rows(a.nx, a.ny, cus) == 15) {
+        const int tiles = step2_tiles_for(a.nx, a.ny, 15);
+        if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles);
+        else hipLaunchKernelGGL((k_step2<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles);
     } else {
-        const int tiles = ntx * ((a.ny + 13) / 14);
-        hipLaunchKernelGGL((k_step2<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles);
+        const int tiles = step2_tiles_for(a.nx, a.ny, 16);
+        if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles);
+        else hipLaunchKernelGGL((k_step2<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles);
     }
-}
-
-// tiles per z plane of the launch (the partials' P)
-static int step2_tiles(int nx, int ny, int cus) {
-    const int ntx = (nx + 63) / 64;
-    auto rounds_cost = [&](int ty) {
-        const int tiles = ntx * ((ny + ty - 3) / (ty - 2));
-        return ((tiles + cus - 1) / cus) * ty;
-    };
-    const int ty = rounds_cost(15) < rounds_cost(16) ? 15 : 16;
-    return ntx * ((ny + ty - 3) / (ty - 2));
 }
 
 static int device_cus(int* cus) {
@@ -508,45 +50,23 @@ static int device_cus(int* cus) {
     return SW_OK;
 }
 
-// force[t][z] = sum over tiles of partials2[t % ring][z][tile] in a fixed order
 __global__ void k_force_reduce2(const double* __restrict__ partials, int ring, int t0, const int* t_dev, int nz, int P,
                                 double* __restrict__ force) {
-    const int s = blockIdx.y, z = blockIdx.x, lane = threadIdx.x;
-    const int t = t0 + (t_dev ? *t_dev : 0) + s;
-    const double* p = partials + (((size_t)(t % ring) * nz + z) * P) * 2;
-    double sx = 0.0, sz = 0.0;
-    for (int k = lane; k < P; k += 64) { sx += p[2 * k]; sz += p[2 * k + 1]; }
-    sx = wave_sum(sx);
-    sz = wave_sum(sz);
-    if (lane == 0) {
-        force[((size_t)t * nz + z) * 2 + 0] = sx;
-        force[((size_t)t * nz + z) * 2 + 1] = sz;
-    }
+    const int t = t0 + (t_dev ? *t_dev : 0) + (int)blockIdx.y;
+    force_row2(partials, ring, t, (int)blockIdx.x, nz, P, (int)threadIdx.x, force);
 }
 
-}  // namespace sw
-
-using namespace sw;
-
-extern "C" {
-
-long long sw_step2_partials_per_step(int nx, int ny, int nz) {
-    if (check_dims(nx, ny, nz)) return -1;
-    int cus = 0;
-    if (device_cus(&cus)) return -1;
-    return (long long)nz * step2_tiles(nx, ny, cus) * 2;
-}
-
-int sw_step2(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
-             int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
-             int ybc, float scale, const float* shift, double* partials, int ring, void* stream) {
+static int step2_entry(bool single, const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
+                       const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
+                       int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
+                       int ring, void* stream) {
     SW_CHECK_ARG(f_in && f_out && links && uin, "f_in, f_out, links and uin must be non-null");
     SW_CHECK_ARG(f_in != f_out, "f_in and f_out must be different buffers (pull scheme)");
     SW_CHECK_RC(check_dims(nx, ny, nz));
     SW_CHECK_RC(check_dtype(dtype, scale));
-    SW_CHECK_ARG(dtype != DT_F64, "sw_step2 takes FP32, FP16 or BF16 storage");
+    SW_CHECK_ARG(dtype != DT_F64, "the two-step kernels take FP32, FP16 or BF16 storage");
     SW_CHECK_ARG((size_t)Q * nx * ny * nz * (dtype == DT_F32 ? 4 : 2) < ((size_t)1 << 32),
-                 "sw_step2 needs population arrays below 4 GiB; use sw_step");
+                 "the two-step kernels need population arrays below 4 GiB; use sw_step");
     SW_CHECK_ARG(ybc == Y_PERIODIC || ybc == Y_FREESLIP, "ybc must be 0 (periodic) or 1 (free-slip)");
     SW_CHECK_ARG(!partials || ring >= 2, "ring must be at least 2 when partials are given");
     SW_CHECK_ARG(t >= 0, "t must be non-negative");
@@ -561,12 +81,47 @@ int sw_step2(const void* f_in, void* f_out, int dtype, const uint32_t* links, co
     a.scale = scale; a.inv_scale = 1.0f / scale;
     hipStream_t s = (hipStream_t)stream;
     switch (dtype) {
-        case DT_F32: launch_step2<float>(a, cus, s); break;
-        case DT_F16: launch_step2<_Float16>(a, cus, s); break;
-        case DT_BF16: launch_step2<bf16_t>(a, cus, s); break;
+        case DT_F32: single ? launch_step2<float, true>(a, cus, s) : launch_step2<float, false>(a, cus, s); break;
+        case DT_F16: single ? launch_step2<_Float16, true>(a, cus, s) : launch_step2<_Float16, false>(a, cus, s); break;
+        case DT_BF16: single ? launch_step2<bf16_t, true>(a, cus, s) : launch_step2<bf16_t, false>(a, cus, s); break;
     }
     SW_HIP(hipGetLastError());
     return SW_OK;
+}
+
+}  // namespace sw
+
+using namespace sw;
+
+extern "C" {
+
+int sw_set_step2_rows(int rows) {
+    SW_CHECK_ARG(rows == 0 || rows == 15 || rows == 16, "rows must be 0 (automatic), 15 or 16");
+    g_step2_rows = rows;
+    return SW_OK;
+}
+
+int sw_get_step2_rows(void) { return g_step2_rows; }
+
+long long sw_step2_partials_per_step(int nx, int ny, int nz) {
+    if (check_dims(nx, ny, nz)) return -1;
+    int cus = 0;
+    if (device_cus(&cus)) return -1;
+    return (long long)nz * step2_tiles_for(nx, ny, step2_rows(nx, ny, cus)) * 2;
+}
+
+int sw_step2(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
+             int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge, double nu_sponge,
+             int ybc, float scale, const float* shift, double* partials, int ring, void* stream) {
+    return step2_entry(false, f_in, f_out, dtype, links, uin, gust, t, t_dev, nx, ny, nz, nu0, cs, x_sponge,
+                       nu_sponge, ybc, scale, shift, partials, ring, stream);
+}
+
+int sw_step2_one(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
+                 int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge,
+                 double nu_sponge, int ybc, float scale, const float* shift, double* partials, int ring, void* stream) {
+    return step2_entry(true, f_in, f_out, dtype, links, uin, gust, t, t_dev, nx, ny, nz, nu0, cs, x_sponge,
+                       nu_sponge, ybc, scale, shift, partials, ring, stream);
 }
 
 int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int nx, int ny, int nz,
@@ -577,7 +132,7 @@ int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev,
     SW_CHECK_ARG(t0 >= 0, "t0 must be non-negative");
     int cus = 0;
     SW_CHECK_RC(device_cus(&cus));
-    const int P = step2_tiles(nx, ny, cus);
+    const int P = step2_tiles_for(nx, ny, step2_rows(nx, ny, cus));
     hipLaunchKernelGGL(k_force_reduce2, dim3(nz, nsteps), dim3(64), 0, (hipStream_t)stream, partials, ring, t0, t_dev,
                        nz, P, force);
     SW_HIP(hipGetLastError());

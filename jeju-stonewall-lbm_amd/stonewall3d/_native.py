@@ -35,7 +35,11 @@ SIGNATURES = {
     "sw_step2_partials_per_step": (_c_ll, [_c_int, _c_int, _c_int]),
     "sw_step2": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
                           _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p]),
+    "sw_step2_one": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
+                              _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p]),
     "sw_force_reduce2": (_c_int, [_p, _c_int, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
+    "sw_set_step2_rows": (_c_int, [_c_int]),
+    "sw_get_step2_rows": (_c_int, []),
     "sw_step_variants": (_c_int, []),
     "sw_set_step_variant": (_c_int, [_c_int, _c_int]),
     "sw_get_step_variant": (_c_int, [_c_int]),
@@ -55,6 +59,12 @@ SIGNATURES = {
     "sw_batch_force_reduce": (_c_int, [_p, _c_int, _c_int, _p]),
     "sw_batch_advance": (_c_int, [_p, _c_int, _c_int, _p]),
     "sw_batch_graph_create": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
+    "sw_batch_partials_per_step": (_c_ll, [_c_int, _c_int, _c_int]),
+    "sw_batch_step2": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "sw_batch_step2_tail": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "sw_batch_force_reduce2": (_c_int, [_p, _c_int, _c_int, _p]),
+    "sw_batch_advance2": (_c_int, [_p, _c_int, _c_int, _p]),
+    "sw_batch_graph_create2": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p]),
     "sw_graph_launch": (_c_int, [_p, _p]),
     "sw_graph_destroy": (_c_int, [_p]),
     "sw_to_ref_layout": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
@@ -69,7 +79,7 @@ class SwInstance(ctypes.Structure):
                 ("force", _p), ("shift", _p), ("nu0", _c_dbl), ("cs", _c_dbl), ("nu_sponge", _c_dbl), ("scale", _c_flt),
                 ("nx", _c_int), ("ny", _c_int), ("nz", _c_int), ("ring", _c_int), ("x_sponge", _c_int),
                 ("ybc", _c_int), ("dtype", _c_int), ("t", _c_int), ("t_end", _c_int), ("parity", _c_int),
-                ("item0", _c_int), ("reserved", _c_int)]
+                ("item0", _c_int), ("tile0", _c_int)]
 
 
 _lib = None

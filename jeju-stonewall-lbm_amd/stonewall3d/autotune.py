@@ -78,8 +78,13 @@ class Autotuner:
         same shapes, or re-initialise the members afterwards."""
         lib = nat.lib()
         cells = sum(m.N for m in batch.members)
-        key = f"batch/{batch.dtype}/{size_class(cells)}/{len(batch.members)}"
+        two = getattr(batch, "two_step", False)
+        key = f"batch/{batch.dtype}/{size_class(cells)}/{len(batch.members)}" + ("/two-step" if two else "")
         cu = torch.cuda.get_device_properties(batch.device).multi_processor_count
+        if two:
+            # the two-step batch kernel has no tile variants, and its workgroup takes the
+            # LDS of a whole compute unit: one per compute unit is the only useful grid
+            variants, per_cu = [batch.variant], (1,)
         if key in self.cache:
             b = self.cache[key]
             if apply:
@@ -87,6 +92,8 @@ class Autotuner:
             return b["ms"]
         variants = list(range(lib.sw_step_variants())) if variants is None else list(variants)
         old = (batch.variant, batch.blocks)
+        if two:
+            variants = [batch.variant]
         res = {}
         for v in variants:
             for p in per_cu:

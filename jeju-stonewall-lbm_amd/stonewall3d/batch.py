@@ -10,6 +10,13 @@ the table, so lattices can join and leave between replays without recapture.
 
 Each lattice's arithmetic is that of the single-lattice step, so its populations and
 force rows are bit-identical alone and in any mix (given the same tile variant).
+
+FP32 batches run the two-step form by default: each launch advances every lattice by two
+steps (sw_batch_step2: 64 x 14-cell tiles of all lattices taken by persistent workgroups
+from a device counter), a tail launch takes the odd last step of lattices that end inside
+the chunk, and the force rows come from the two-step per-tile partials.  A lattice's
+populations are bit-identical to single steps; its force rows are bit-identical to a
+Simulation(two_step=True) run with sw_set_step2_rows(16).
 """
 import ctypes
 
@@ -58,8 +65,9 @@ class Lattice:
                 raise ValueError(f"gust table has {g.shape[0] if g.ndim else 0} entries; the lattice's "
                                  f"{self.n_steps} steps read one per step")
             self.gust = torch.from_numpy(g[:self.n_steps].copy()).to(dev)
-        self.partials = torch.zeros(self.ring * lib.sw_partials_per_step(self.ny, self.nz), dtype=torch.float64,
-                                    device=dev)
+        # room for either partials layout (one-step row groups or two-step tiles)
+        self.partials = torch.zeros(self.ring * lib.sw_batch_partials_per_step(self.nx, self.ny, self.nz),
+                                    dtype=torch.float64, device=dev)
         self.force = torch.zeros((self.n_steps, self.nz, 2), dtype=torch.float64, device=dev)
         self.t = 0
         self.parity = 0
@@ -121,6 +129,11 @@ def default_blocks(device=None, per_cu=8):
     return int(props.multi_processor_count) * per_cu
 
 
+def two_step_eligible(lattice):
+    """The two-step form takes FP32 / FP16 / BF16 population arrays below 4 GiB."""
+    return lattice.dtype != nat.SW_F64 and Q * lattice.N * nat.DTYPE_BYTES[lattice.dtype] < (1 << 32)
+
+
 class Batch:
     """A set of lattices of one dtype advanced K steps per launch sequence.
 
@@ -132,14 +145,22 @@ class Batch:
                lattice size (rows of 128-640 cells), else the library's current choice for
                this dtype.
     graph    : capture the chunk as a hipGraph (else launch it eagerly).
+    two_step : advance two steps per launch (sw_batch_step2; K even, FP32 / FP16 / BF16
+               lattices below 4 GiB per population array); default: on for FP32, whose
+               two-step kernel is the faster one (the 16-bit forms are faster one step per
+               launch).  `blocks` then defaults to one workgroup per compute unit.
     """
 
-    def __init__(self, dtype="f32", max_instances=64, K=16, blocks=None, variant=None, graph=True, device="cuda"):
+    def __init__(self, dtype="f32", max_instances=64, K=16, blocks=None, variant=None, graph=True, device="cuda",
+                 two_step=None):
         lib = nat.lib()
         self.dtype = nat.DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
         self.max_instances, self.K = int(max_instances), int(K)
         self.device = torch.device(device)
-        self.blocks = int(blocks) if blocks else default_blocks(self.device)
+        self.two_step = (self.dtype == nat.SW_F32) if two_step is None else bool(two_step)
+        if self.two_step and (self.dtype == nat.SW_F64 or self.K % 2):
+            raise ValueError("the two-step batch needs FP32 / FP16 / BF16 storage and an even K")
+        self.blocks = int(blocks) if blocks else default_blocks(self.device, per_cu=1 if self.two_step else 8)
         if variant is None:
             variant = 8 if self.dtype in (nat.SW_F16, nat.SW_BF16) else lib.sw_get_step_variant(self.dtype)
         self.variant = int(variant)
@@ -179,6 +200,8 @@ class Batch:
             raise ValueError(f"lattice ring {lattice.ring} < K={self.K}")
         if len(self.members) >= self.max_instances:
             raise RuntimeError("batch is full")
+        if self.two_step and not two_step_eligible(lattice):
+            raise ValueError("the two-step batch takes population arrays below 4 GiB")
         self.sync()
         self.members.append(lattice)
         self._upload()
@@ -199,13 +222,17 @@ class Batch:
 
     # ---------------------------------------------------------------- stepping
     def _ensure_graph(self):
-        key = (self.dtype, self.K, self.blocks, self.variant, self.big)
+        key = (self.dtype, self.K, self.blocks, self.variant, self.big, self.two_step)
         if self._graph is not None and self._graph_key == key:
             return
         self.close()
         h = ctypes.c_void_p()
-        nat.call("sw_batch_graph_create", self.table.data_ptr(), self.max_instances, self.dtype, self.K, self.blocks,
-                 self.variant, int(self.big), ctypes.byref(h))
+        if self.two_step:
+            nat.call("sw_batch_graph_create2", self.table.data_ptr(), self.max_instances, self.dtype, self.K,
+                     self.blocks, ctypes.byref(h))
+        else:
+            nat.call("sw_batch_graph_create", self.table.data_ptr(), self.max_instances, self.dtype, self.K,
+                     self.blocks, self.variant, int(self.big), ctypes.byref(h))
         self._graph, self._graph_key = h, key
 
     def run_chunk(self):
@@ -216,6 +243,12 @@ class Batch:
         if self.use_graph:
             self._ensure_graph()
             nat.call("sw_graph_launch", self._graph, s)
+        elif self.two_step:
+            for p in range(self.K // 2):
+                nat.call("sw_batch_step2", self.table.data_ptr(), self.dtype, p, self.blocks, s)
+            nat.call("sw_batch_step2_tail", self.table.data_ptr(), self.dtype, self.K, self.blocks, s)
+            nat.call("sw_batch_force_reduce2", self.table.data_ptr(), self.max_instances, self.K, s)
+            nat.call("sw_batch_advance2", self.table.data_ptr(), self.max_instances, self.K, s)
         else:
             for k in range(self.K):
                 nat.call("sw_batch_step", self.table.data_ptr(), self.dtype, k, self.blocks, self.variant,
@@ -266,10 +299,12 @@ def main(argv=None):
     ap.add_argument("--K", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--one-step", action="store_true",
+                    help="one step per launch (default for FP32: two, sw_batch_step2)")
     a = ap.parse_args(argv)
     spec = workloads.batch_instances(a.instances, seed=a.seed, n_steps=a.steps)
     cells = 0
-    b = Batch(a.dtype, max_instances=a.instances, K=a.K)
+    b = Batch(a.dtype, max_instances=a.instances, K=a.K, two_step=False if a.one_step else None)
     for H, por, seed, steps in spec:
         w = workloads.porous_wall(H=H, porosity=por, seed=seed, n_steps=steps)
         b.add(Lattice(w.flag, w.uin, dtype=a.dtype, ring=a.K, **w.sim_kwargs()).init(noise=0.02, seed=seed))
@@ -303,7 +338,8 @@ def main(argv=None):
     bpu = {"f32": 152, "f16": 76, "bf16": 76, "f64": 304}[a.dtype]
     out = dict(form="batched", dtype=a.dtype, instances=a.instances, cells=cells, steps=a.steps, seconds=el,
                aggregate_mlups=cells * a.steps / el / 1e6, achieved_gbs=cells * a.steps * bpu / el / 1e9,
-               variant=b.variant, workgroups=b.blocks, K=a.K, graph=b.use_graph, finite=finite,
+               variant=None if b.two_step else b.variant, two_step=b.two_step, workgroups=b.blocks, K=a.K,
+               graph=b.use_graph, finite=finite,
                tuning_ms_per_step={f"{k[0]}/{k[1]}": v for k, v in tuned.items()} if tuned else None)
     print(json.dumps(out), flush=True)
     b.close()

@@ -89,8 +89,10 @@ class Simulation:
     n_steps   : steps this run may take (sizes the force record and the gust table).
     ring      : steps per force-partials ring (reduced every `ring` steps).
     two_step  : advance pairs of steps with one launch of the two-step kernel (sw_step2:
-                FP32 / FP16 / BF16 storage, population arrays below 4 GiB); populations are
-                bit-identical to single steps, force rows equal to rounding.
+                FP32 / FP16 / BF16 storage, population arrays below 4 GiB) and odd steps with
+                its one-step form (sw_step2_one); populations are bit-identical to single
+                steps, force rows equal to rounding (to the two-step batch bit for bit with
+                sw_set_step2_rows(16)).
     """
 
     def __init__(self, flag, uin, *, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, ybc="periodic", dtype="f32",
@@ -182,12 +184,22 @@ class Simulation:
             self.Uacc = torch.zeros(4 * self.N, dtype=torch.float32, device=self.device)
         while n > 0:
             pair = self.two_step and (n >= 3 or (n == 2 and not accumulate))
-            layout = 2 if pair else 1
+            # with two_step, a single step other than the accumulating one takes the two-step
+            # kernel's tiles (sw_step2_one), so its partials share the pairs' ring layout
+            single2 = self.two_step and not pair and not (accumulate and n == 1)
+            layout = 2 if (pair or single2) else 1
+            adv = 2 if pair else 1
             if self.record_force and self._pending and (self._layout != layout or
-                                                        self._pending + layout > self.ring):
+                                                        self._pending + adv > self.ring):
                 self.flush_force()
             self._layout = layout
-            if pair:
+            if single2:
+                rc = lib.sw_step2_one(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
+                                      self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
+                                      self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
+                                      _ptr(self.shift), _ptr(self.partials2), self.ring, s)
+                nat.check(rc, "sw_step2_one")
+            elif pair:
                 rc = lib.sw_step2(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
                                   self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
                                   self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
@@ -208,10 +220,10 @@ class Simulation:
                                  _ptr(self.shift), _ptr(self.partials), self.ring, s)
                 nat.check(rc, "sw_step")
             self.f[0], self.f[1] = self.f[1], self.f[0]
-            self.t += layout
-            n -= layout
+            self.t += adv
+            n -= adv
             if self.record_force:
-                self._pending += layout
+                self._pending += adv
                 if self._pending == self.ring:
                     self.flush_force()
         return self

@@ -74,7 +74,9 @@ def lattice_bytes(H, dtype, steps, ring):
     e = nat.DTYPE_BYTES[nat.DTYPES[dtype]]
     links = N + ny * nz * ((nx + 2047) // 2048)
     nyb = (ny + 3) // 4
-    return 2 * 19 * N * e + 4 * links + 8 * ring * nz * nyb * 2 + 16 * steps * nz + 4 * (nz + steps)
+    tiles = ((nx + 63) // 64) * ((ny + 13) // 14)         # the two-step batch's 64 x 14-cell tiles
+    partials = 8 * ring * nz * max(nyb, tiles) * 2
+    return 2 * 19 * N * e + 4 * links + partials + 16 * steps * nz + 4 * (nz + steps)
 
 
 class Server:
@@ -87,7 +89,7 @@ class Server:
     keep_state = False
 
     def __init__(self, budget_bytes=250e9, K=32, max_instances=64, graph=True, keep_force=False, noise=0.02,
-                 device="cuda", variants=None, policy="fifo", fill_cells=None, keep_state=False):
+                 device="cuda", variants=None, policy="fifo", fill_cells=None, keep_state=False, two_step=None):
         if policy not in ("fifo", "sjf"):
             raise ValueError("policy must be 'fifo' or 'sjf'")
         self.policy = policy
@@ -101,6 +103,7 @@ class Server:
         self.noise = float(noise)
         self.device = torch.device(device)
         self.variants = dict(variants or {})
+        self.two_step = two_step           # None: the Batch default (two steps per launch for FP32)
         self.blocks = {}
         self.batches = {}
         self.streams = {}
@@ -114,7 +117,8 @@ class Server:
             self.streams[dtype] = torch.cuda.Stream(device=self.device)
             with torch.cuda.stream(self.streams[dtype]):
                 self.batches[dtype] = Batch(dtype, self.max_instances, self.K, blocks=self.blocks.get(dtype),
-                                            graph=self.graph, device=self.device, variant=self.variants.get(dtype))
+                                            graph=self.graph, device=self.device, variant=self.variants.get(dtype),
+                                            two_step=self.two_step)
         return self.batches[dtype]
 
     def make_lattice(self, req):
@@ -142,7 +146,8 @@ class Server:
         tuner = tuner or Autotuner()
         out = {}
         for d in dtypes:
-            b = Batch(d, max_instances=len(heights), K=self.K, graph=self.graph, device=self.device)
+            b = Batch(d, max_instances=len(heights), K=self.K, graph=self.graph, device=self.device,
+                      two_step=self.two_step)
             for h in heights:
                 w = workloads.porous_wall(H=h, porosity=0.2, seed=h, n_steps=4 * self.K)
                 b.add(Lattice(w.flag, w.uin, dtype=d, ring=self.K, device=self.device, **w.sim_kwargs())

@@ -20,6 +20,12 @@ def wl(H, por, seed, steps):
     return workloads.porous_wall(H=H, porosity=por, seed=seed, n_steps=steps)
 
 
+def lat_flag(lat):
+    """The lattice's cell flags in the reference layout (nx, ny, nz), from its link masks."""
+    m = lat.links[:lat.N].cpu().numpy().view(np.uint32).reshape(lat.nz, lat.ny, lat.nx)
+    return ((m >> 31) & 1).astype(np.uint8).transpose(2, 1, 0)
+
+
 def alone(w, dtype, variant, seed, shift=None):
     code = nat.DTYPES[dtype]
     old = nat.lib().sw_get_step_variant(code)
@@ -30,6 +36,20 @@ def alone(w, dtype, variant, seed, shift=None):
         return sim.populations(), sim.force_rows()
     finally:
         nat.call("sw_set_step_variant", code, old)
+
+
+def alone2(w, dtype, seed, shift=None):
+    """The lattice stepped alone with the two-step kernels on the two-step batch's tiles
+    (16 rows): pairs by sw_step2, an odd last step by sw_step2_one."""
+    old = nat.lib().sw_get_step2_rows()
+    nat.call("sw_set_step2_rows", 16)
+    try:
+        sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, two_step=True, **w.sim_kwargs())
+        sim.init(noise=0.02, seed=seed)
+        sim.step(w.n_steps)
+        return sim.populations(), sim.force_rows()
+    finally:
+        nat.call("sw_set_step2_rows", old)
 
 
 def lattice(w, dtype, seed, K, shift=None):
@@ -43,7 +63,7 @@ def test_alone_equals_mix(gpu, dtype, shift, variant):
     K = 8
     ws = [wl(*c) for c in CASES]
     ref = [alone(w, dtype, variant, c[2], shift) for w, c in zip(ws, CASES)]
-    b = Batch(dtype, max_instances=8, K=K, variant=variant, graph=True)
+    b = Batch(dtype, max_instances=8, K=K, variant=variant, graph=True, two_step=False)
     lats = [lattice(w, dtype, c[2], K, shift) for w, c in zip(ws, CASES)]
     for lat in lats:
         b.add(lat)
@@ -83,7 +103,7 @@ def test_alone_equals_mix_ragged_default_tiles(gpu, dtype):
     ws = ragged_workloads()
     seeds = [21, 22, 23, 24]
     ref = [alone(w, dtype, variant, s) for w, s in zip(ws, seeds)]
-    b = Batch(dtype, max_instances=8, K=K, variant=variant, graph=True)
+    b = Batch(dtype, max_instances=8, K=K, variant=variant, graph=True, two_step=False)
     lats = [lattice(w, dtype, s, K) for w, s in zip(ws, seeds)]
     for lat in lats:
         b.add(lat)
@@ -95,12 +115,60 @@ def test_alone_equals_mix_ragged_default_tiles(gpu, dtype):
     b.close()
 
 
-def test_graph_replays_equal_eager(gpu):
+@pytest.mark.parametrize("dtype,shift", [("f32", None), ("f16", "inlet"), ("bf16", "inlet")])
+def test_two_step_batch_alone_equals_mix(gpu, dtype, shift):
+    """The two-step batch (pairs of steps per launch, the odd last step in the tail launch):
+    each lattice's populations equal its single steps (the one-step run alone) and its force
+    rows equal the two-step run alone on the same tiles; step counts 37 / 40 / 29 end
+    inside chunks of 8 with odd and even remainders."""
+    K = 8
+    ws = [wl(*c) for c in CASES]
+    ref1 = [alone(w, dtype, 9, c[2], shift) for w, c in zip(ws, CASES)]
+    ref2 = [alone2(w, dtype, c[2], shift) for w, c in zip(ws, CASES)]
+    b = Batch(dtype, max_instances=8, K=K, graph=True, two_step=True)
+    lats = [lattice(w, dtype, c[2], K, shift) for w, c in zip(ws, CASES)]
+    for lat in lats:
+        b.add(lat)
+    b.run_to_completion()
+    for lat, (p1, F1), (p2, F2) in zip(lats, ref1, ref2):
+        assert lat.t == lat.n_steps
+        pb = lat.populations()
+        assert np.array_equal(pb, p2)
+        fluid = (lat_flag(lat) == 0)[None]
+        assert np.array_equal(np.where(fluid, pb, 0), np.where(fluid, p1, 0))
+        assert np.array_equal(lat.force_rows(), F2)
+        scale = np.abs(F1).max()
+        assert np.allclose(lat.force_rows(), F1, rtol=1e-12, atol=1e-15 * scale)
+    b.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+def test_two_step_batch_ragged(gpu, dtype):
+    """Ragged lattices (2x1x2 up to a porous wall, both side boundaries) in the two-step
+    batch: identical to the two-step run alone."""
+    K = 8
+    ws = ragged_workloads()
+    seeds = [21, 22, 23, 24]
+    ref = [alone2(w, dtype, s) for w, s in zip(ws, seeds)]
+    b = Batch(dtype, max_instances=8, K=K, graph=True, two_step=True)
+    lats = [lattice(w, dtype, s, K) for w, s in zip(ws, seeds)]
+    for lat in lats:
+        b.add(lat)
+    b.run_to_completion()
+    for lat, (p, F) in zip(lats, ref):
+        assert lat.t == lat.n_steps
+        assert np.array_equal(lat.populations(), p)
+        assert np.array_equal(lat.force_rows(), F)
+    b.close()
+
+
+@pytest.mark.parametrize("two_step", [False, True])
+def test_graph_replays_equal_eager(gpu, two_step):
     K = 8
     ws = [wl(*c) for c in CASES]
     outs = []
     for graph in (True, False):
-        b = Batch("f32", max_instances=8, K=K, variant=VARIANT, graph=graph)
+        b = Batch("f32", max_instances=8, K=K, variant=VARIANT, graph=graph, two_step=two_step)
         lats = [lattice(w, "f32", c[2], K) for w, c in zip(ws, CASES)]
         for lat in lats:
             b.add(lat)
@@ -142,11 +210,12 @@ def test_batch_with_lattice_beyond_4gib(gpu):
     b.close()
 
 
-def test_join_mid_run(gpu):
+@pytest.mark.parametrize("two_step", [False, True])
+def test_join_mid_run(gpu, two_step):
     K = 8
     ws = [wl(*c) for c in CASES]
-    ref = [alone(w, "f32", VARIANT, c[2]) for w, c in zip(ws, CASES)]
-    b = Batch("f32", max_instances=8, K=K, variant=VARIANT, graph=True)
+    ref = [alone2(w, "f32", c[2]) if two_step else alone(w, "f32", VARIANT, c[2]) for w, c in zip(ws, CASES)]
+    b = Batch("f32", max_instances=8, K=K, variant=VARIANT, graph=True, two_step=two_step)
     lats = [lattice(w, "f32", c[2], K) for w, c in zip(ws, CASES)]
     b.add(lats[0])
     b.run(2)
@@ -165,6 +234,8 @@ def test_join_mid_run(gpu):
 
 
 def test_server_results_alone_equal_mix(gpu):
+    """Served results equal single runs: FP32 requests take the two-step batch (the Batch
+    default), FP16 requests the one-step batch with the given tile."""
     from stonewall3d.serving import Request, Server
     reqs = [Request(0, 0.0, 6, 0.2, 21, 33, "f32"), Request(1, 0.0, 5, 0.4, 22, 41, "f16"),
             Request(2, 0.0, 7, 0.1, 23, 26, "f32"), Request(3, 0.0, 6, 0.3, 24, 50, "f16")]
@@ -175,7 +246,7 @@ def test_server_results_alone_equal_mix(gpu):
     for r in results:
         req = reqs[r.rid]
         w = workloads.porous_wall(H=req.H, porosity=req.porosity, seed=req.seed, n_steps=req.steps)
-        _, F = alone(w, req.dtype, VARIANT, req.seed)
+        _, F = alone2(w, req.dtype, req.seed) if req.dtype == "f32" else alone(w, req.dtype, VARIANT, req.seed)
         assert np.array_equal(r.force, F), f"request {r.rid} differs in the mix"
 
 

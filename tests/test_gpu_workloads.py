@@ -107,8 +107,9 @@ def test_c2_mean_cd_within_1pct_of_fp64(gpu, c2_fp64_run, storage):
 def test_c3_batch_of_64_alone_equals_mix(gpu, dtype):
     """The batched configuration's own mix (workloads.batch_instances(64): 64 lattices
     16H x 8H x 4H, H from {8, 12, 16, 20, 24}, 210.9 M cells): two 16-step graph chunks of
-    the persistent batch kernel, then every lattice's population buffer and force rows
-    equal those of the same lattice stepped alone."""
+    the persistent batch kernel (FP32: the two-step form, FP16: one step per launch), then
+    every lattice's population buffer and force rows equal those of the same lattice
+    stepped alone with the same kernels and tiles."""
     from stonewall3d.batch import Batch, Lattice
     from stonewall3d.lbm3d import Simulation
     K, steps = 16, 32
@@ -120,22 +121,27 @@ def test_c3_batch_of_64_alone_equals_mix(gpu, dtype):
         lat = Lattice(w.flag, w.uin, dtype=dtype, ring=K, **w.sim_kwargs()).init(noise=0.02, seed=seed)
         b.add(lat)
         lats.append((lat, w, seed))
-    print(f"c3 {dtype}: {sum(l.N for l, _, _ in lats) / 1e6:.1f} M cells, variant {b.variant}", flush=True)
+    print(f"c3 {dtype}: {sum(l.N for l, _, _ in lats) / 1e6:.1f} M cells, two-step {b.two_step}, variant {b.variant}",
+          flush=True)
     b.run(2)
     torch.cuda.synchronize()
     code = nat.DTYPES[dtype]
     old = nat.lib().sw_get_step_variant(code)
+    old_rows = nat.lib().sw_get_step2_rows()
     nat.call("sw_set_step_variant", code, b.variant)
+    nat.call("sw_set_step2_rows", 16)
     try:
         for lat, w, seed in lats:
             assert lat.t == steps
-            sim = Simulation(w.flag, w.uin, dtype=dtype, **w.sim_kwargs()).init(noise=0.02, seed=seed)
+            sim = Simulation(w.flag, w.uin, dtype=dtype, two_step=b.two_step, **w.sim_kwargs())
+            sim.init(noise=0.02, seed=seed)
             sim.step(steps)
             assert torch.equal(lat.current(), sim.f[0]), f"lattice {w.name} differs in the mix"
             assert np.array_equal(lat.force_rows(), sim.force_rows())
             del sim
     finally:
         nat.call("sw_set_step_variant", code, old)
+        nat.call("sw_set_step2_rows", old_rows)
     b.close()
     del b, lats
     torch.cuda.empty_cache()
@@ -159,16 +165,21 @@ def test_c4_serving_trace_alone_equals_mix(gpu):
         req = reqs[r.rid]
         w = workloads.porous_wall(H=req.H, porosity=req.porosity, seed=req.seed, n_steps=req.steps)
         code = nat.DTYPES[req.dtype]
+        bt = srv.batches[req.dtype]
         old = nat.lib().sw_get_step_variant(code)
-        nat.call("sw_set_step_variant", code, srv.batches[req.dtype].variant)
+        old_rows = nat.lib().sw_get_step2_rows()
+        nat.call("sw_set_step_variant", code, bt.variant)
+        nat.call("sw_set_step2_rows", 16)
         try:
-            sim = Simulation(w.flag, w.uin, dtype=req.dtype, **w.sim_kwargs()).init(noise=0.02, seed=req.seed)
+            sim = Simulation(w.flag, w.uin, dtype=req.dtype, two_step=bt.two_step, **w.sim_kwargs())
+            sim.init(noise=0.02, seed=req.seed)
             sim.step(req.steps)
             assert torch.equal(r.state, sim.f[0]), f"request {r.rid} (H={req.H}, {req.dtype}) differs in the mix"
             assert np.array_equal(r.force, sim.force_rows())
             del sim
         finally:
             nat.call("sw_set_step_variant", code, old)
+            nat.call("sw_set_step2_rows", old_rows)
     del results
     torch.cuda.empty_cache()
 

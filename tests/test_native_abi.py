@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_sizes():
     L = nat.lib()
-    assert L.sw_version() == 110
+    assert L.sw_version() == 120
     assert L.sw_rows_per_block() == 4
     assert L.sw_partials_per_step(320, 160) == 160 * 80 * 2
     assert L.sw_partials_per_step(321, 3) == 3 * 81 * 2
@@ -83,3 +83,36 @@ def test_check_raises_with_message():
     with pytest.raises(nat.SwError) as e:
         nat.call("sw_macro", None, 0, None, 8, 8, 8, 1.0, None, None, None, None)
     assert e.value.code == 1001
+
+
+def test_two_step_sizes_and_argument_errors_without_device():
+    """Host-side rules of the two-step entry points: partials sizes, the tile-rows setting,
+    the batch table size, and argument errors raised before any device work."""
+    L = nat.lib()
+    # the two-step batch's tiles: 64 x 14 cells; a lattice's ring holds either layout
+    assert L.sw_batch_partials_per_step(640, 320, 160) == 160 * (10 * 23) * 2
+    assert L.sw_batch_partials_per_step(128, 64, 32) == 32 * 16 * 2          # the row groups are more
+    assert L.sw_batch_partials_per_step(0, 8, 8) == -1
+    inst = ctypes.sizeof(nat.SwInstance)
+    assert inst % 8 == 0
+    assert L.sw_batch_table_bytes(4) == 64 + 4 * (inst + 2 * 136)           # + two argument records each
+    # tile rows: 0 (automatic), 15 or 16
+    assert L.sw_get_step2_rows() == 0
+    assert L.sw_set_step2_rows(17) == 1001
+    assert b"15 or 16" in L.sw_last_error()
+    assert L.sw_set_step2_rows(16) == 0 and L.sw_get_step2_rows() == 16
+    assert L.sw_set_step2_rows(0) == 0 and L.sw_get_step2_rows() == 0
+    fake = ctypes.c_void_p(0x1000)
+    args = (0, fake, fake, None, 0, None, 8, 8, 8, 1e-3, 0.16, 8, 0.05, 0, 1.0, None, None, 2, None)
+    assert L.sw_step2_one(None, ctypes.c_void_p(0x2000), *args) == 1001
+    assert L.sw_step2_one(fake, fake, *args) == 1001                         # f_in == f_out
+    f64 = (3,) + args[1:]
+    assert L.sw_step2_one(fake, ctypes.c_void_p(0x2000), *f64) == 1001
+    assert b"FP32, FP16 or BF16" in L.sw_last_error()
+    assert L.sw_batch_step2(None, 0, 0, 256, None) == 1001
+    assert L.sw_batch_step2(fake, 3, 0, 256, None) == 1001                   # FP64
+    assert L.sw_batch_step2_tail(fake, 0, 7, 256, None) == 1001              # K odd
+    assert L.sw_batch_advance2(fake, 4, 7, None) == 1001
+    g = ctypes.c_void_p()
+    assert L.sw_batch_graph_create2(fake, 4, 0, 7, 256, ctypes.byref(g)) == 1001
+    assert b"even" in L.sw_last_error()
