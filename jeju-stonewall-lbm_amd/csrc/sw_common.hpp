@@ -171,6 +171,16 @@ __host__ __device__ __forceinline__ float neg_scaled(float s, int k) {
 }
 
 // ------------------------------------------------------------- memory access
+// A load of data that no kernel writes while a step runs (chunk flags, inlet profile, gust and
+// storage-offset tables), through the constant address space: at a wave-uniform address the
+// compiler then issues a scalar load, and a test of its value waits only for that load
+// (lgkmcnt), not for every vector load and store still in flight (vmcnt(0), as a global load
+// of the same word needed).
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p, size_t i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // Raw-buffer access: one descriptor per array, a wave-uniform 32-bit byte offset (row
 // start, scalar register) plus a per-lane 32-bit byte offset.  Usable while every byte
 // offset of the array fits in 32 bits.

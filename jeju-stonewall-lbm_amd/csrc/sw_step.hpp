@@ -349,7 +349,7 @@ __device__ __forceinline__ void step_row(A& a, int y, int z, int t, int lane, in
 #pragma unroll
         for (int l = 0; l < 3; l++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) sg[l][c] = a.shift ? (T)a.shift[zl[l] * 4 + c] : T(1);
+            for (int c = 0; c < 3; c++) sg[l][c] = a.shift ? (T)ld_const(a.shift, zl[l] * 4 + c) : T(1);
 #pragma unroll
         for (int c = 0; c < 3; c++) en[c] = (T)neg_scaled((float)sg[1][c], kscale);
     }
@@ -387,7 +387,7 @@ __device__ __forceinline__ void step_row(A& a, int y, int z, int t, int lane, in
     const uint32_t seglen = (nch_row + TL::NSEG - 1) / TL::NSEG;
     const uint32_t cbeg = min(nch_row, (uint32_t)seg * seglen);
     const uint32_t nchunks = min(nch_row, cbeg + seglen);        // end of this wavefront's range
-    auto special_chunk = [&](uint32_t c) -> bool { return (rflags[c >> 5] >> (c & 31)) & 1u; };
+    auto special_chunk = [&](uint32_t c) -> bool { return (ld_const(rflags, c >> 5) >> (c & 31)) & 1u; };
 
     auto load_fast = [&](uint32_t x, T (&fl)[Q]) {
         const uint32_t xo = x * ES;
@@ -565,8 +565,8 @@ __device__ __forceinline__ void step_row(A& a, int y, int z, int t, int lane, in
                 static_for<0, Q>([&](auto I) { fl[I] = C::decode((uint16_t)(rl[I] & 0xFFFFu), dsh(I), inv_scale); });
                 collide<NORM, T>(fl, xq, a, smag, [&](auto I, T v) { pend[I] = C::first(v, esh(I), scale); });
                 if (x0 == 0) {                                           // inlet cell x = 0 (:67-71)
-                    T u0 = (T)a.uin[z];
-                    if (a.gust) u0 = u0 * (T)a.gust[t];
+                    T u0 = (T)ld_const(a.uin, z);
+                    if (a.gust) u0 = u0 * (T)ld_const(a.gust, t);
                     const T usq15 = T(1.5) * (u0 * u0);
                     static_for<0, Q>([&](auto I) {
                         constexpr int i = I;
@@ -683,8 +683,8 @@ __device__ __forceinline__ void step_row(A& a, int y, int z, int t, int lane, in
                 collide<NORM, T>(fl, xa, a, smag, [&](auto I, T v) { pnd[I] = C::first(v, esh(I), scale); });
             }
             if (x0 == 0) {                                               // inlet cell x = 0 (:67-71)
-                T u0 = (T)a.uin[z];
-                if (a.gust) u0 = u0 * (T)a.gust[t];
+                T u0 = (T)ld_const(a.uin, z);
+                if (a.gust) u0 = u0 * (T)ld_const(a.gust, t);
                 const T usq15 = T(1.5) * (u0 * u0);
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
@@ -805,8 +805,8 @@ __device__ __forceinline__ void step_row(A& a, int y, int z, int t, int lane, in
             uint32_t oa[Q];
             collide<NORM, T>(fa, xa, a, smag, [&](auto I, T v) { oa[I] = C::encode(v, esh(I), scale); });
             if (x0 == 0) {                                               // inlet cell x = 0 (:67-71)
-                T u0 = (T)a.uin[z];
-                if (a.gust) u0 = u0 * (T)a.gust[t];
+                T u0 = (T)ld_const(a.uin, z);
+                if (a.gust) u0 = u0 * (T)ld_const(a.gust, t);
                 const T usq15 = T(1.5) * (u0 * u0);
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
@@ -870,8 +870,8 @@ __device__ __forceinline__ void step_row(A& a, int y, int z, int t, int lane, in
     }
     // ---------------- inlet (:67-71): x = 0 fluid cell gets feq(1, (uin[z] g, 0, 0))
     if (seg == 0 && lane == 0 && !inlet_done && !(lrow[0] & SOLID_BIT)) {
-        T u0 = (T)a.uin[z];
-        if (a.gust) u0 = u0 * (T)a.gust[t];
+        T u0 = (T)ld_const(a.uin, z);
+        if (a.gust) u0 = u0 * (T)ld_const(a.gust, t);
         const T usq15 = T(1.5) * (u0 * u0);
         T sv[Q];
         static_for<0, Q>([&](auto I) {

@@ -115,7 +115,7 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
     const bool outlet = x == nx - 1;
 
     // offsets of the 16-bit storage: sigma_c(z) of the c_x class c at height z
-    auto sig = [&](int z, int c) -> T { return a.shift ? (T)a.shift[z * 4 + c] : T(1); };
+    auto sig = [&](int z, int c) -> T { return a.shift ? (T)ld_const(a.shift, z * 4 + c) : T(1); };
 
     // byte offset of link I's source row at plane k (lbm3d.py:77-86 with the side rules)
     auto src_off = [&](auto I, int k, bool lo, bool hi, int yrm, int yr0, int yrp) -> uint32_t {
@@ -168,11 +168,11 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         }
     };
     auto chunk_fast = [&](int row, int k, int c) -> bool {
-        return !((cflags[((uint32_t)k * ny + row) * cw + (c >> 5)] >> (c & 31)) & 1u);
+        return !((ld_const(cflags, ((uint32_t)k * ny + row) * cw + (c >> 5)) >> (c & 31)) & 1u);
     };
     auto inlet_u0 = [&](int k, int t) -> T {
-        T u0 = (T)a.uin[k];
-        if (a.gust) u0 = u0 * (T)a.gust[t];
+        T u0 = (T)ld_const(a.uin, k);
+        if (a.gust) u0 = u0 * (T)ld_const(a.gust, t);
         return u0;
     };
     auto wall_force = [&](auto I, T f, T& fxw, T& fzw) {
@@ -255,14 +255,17 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         }
         const uint32_t hxo = (uint32_t)hx * ES, hxm = (uint32_t)(hx > 0 ? hx - 1 : 0) * ES,
                        hxp = (uint32_t)(hx + 1 < nx ? hx + 1 : nx - 1) * ES;
-        const uint32_t hmask = a.links[((uint32_t)kh * ny + hy) * nx + hx];
+        // the streamed values first, then the cell's link mask: the mask's wait then covers
+        // them all (a mask loaded first had its own wait before the other loads were issued)
         B hraw[Q];
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             hraw[i] = buf_load<B>(rin, src_off(I, kh, hlo, hhi, hym, hy, hyp) + (CX[i] < 0 ? hxp : (CX[i] > 0 ? hxm : hxo)),
                                   0u);
         });
-        if (hmask & SOLID_BIT) return;                   // a solid cell's populations are never read
+        const uint32_t hmask = a.links[((uint32_t)kh * ny + hy) * nx + hx];
+        // a solid cell's values are never read (its neighbours bounce back): it writes nothing
+        const bool hlive = !(hmask & SOLID_BIT);
         T fl[Q];
         static_for<0, Q>([&](auto I) { fl[I] = Cd::decode(hraw[I], dsh(I, kh), inv_scale); });
         T fxw = T(0), fzw = T(0);
@@ -270,7 +273,7 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         collide<NORM, T>(fl, (uint32_t)hx, a, smag, [&](auto I, T val) {
             constexpr int i = I;
             if constexpr (CX[i] != 0) {
-                if ((CX[i] > 0) == (hside == 0)) {       // x0-1 feeds c_x = +1, x0+64 feeds c_x = -1
+                if (hlive && (CX[i] > 0) == (hside == 0)) {   // x0-1 feeds c_x = +1, x0+64 feeds c_x = -1
                     const T p = stored(I, val, kh);
                     if constexpr (CY[i] == 0) L.H[kh & 3][s2_hidx(i)][hr] = (float)p;
                     else L.B[kh % 3][s2_gidx(i)][hr][hside ? 65 : 0] = (float)p;

@@ -481,9 +481,9 @@ def main(argv=None):
     if a.max_steps is not None:
         n_total = min(n_total, a.max_steps)
     bpu = BYTES_PER_UPDATE[nat.DTYPES[a.dtype]]
-    print(f"[grid] {nx}x{ny}x{nz} = {nx*ny*nz/1e6:.1f} M cells, ~{2*nx*ny*nz*bpu/2/1e9:.1f} GB populations | "
-          f"U_H={su.UH:.4f} nu={su.nu0:.2e} tau0={3*su.nu0+0.5:.5f}  Re={a.Re:.0f}  steps={n_total} "
-          f"(H/U = {su.t_unit:.0f} steps) dtype={a.dtype}", flush=True)
+    print(f"[lattice] {nx} x {ny} x {nz} ({nx*ny*nz/1e6:.1f} M cells), {a.dtype} populations "
+          f"{2*nx*ny*nz*bpu/2/1e9:.1f} GB; inflow U_H {su.UH:.4f}, viscosity {su.nu0:.2e} (tau0 "
+          f"{3*su.nu0+0.5:.5f}), Re {a.Re:.0f}; {n_total} steps, one H/U = {su.t_unit:.0f} steps", flush=True)
     wall, phi = build_wall(a.porosity, H, nx, ny, nz, X0, T, seed=a.seed)
     gust = gust_table(make_gust(a.gust_seed), n_total) if a.gust_seed is not None else None
     # two steps per launch where it is the faster kernel: FP32 below 4 GiB per array (0.78 against
@@ -504,7 +504,7 @@ def main(argv=None):
     if os.path.isdir(ckpt) and not a.no_resume:
         t_start, n_acc, snaps = checkpoint.load(sim, ckpt, sig)
         snaps_xz, snaps_xy, snaps_q = list(snaps["snaps_xz"]), list(snaps["snaps_xy"]), list(snaps["snaps_q"])
-        print(f"[{a.case}] resumed from step {t_start} ({t_start/su.t_unit:.1f} H/U)", flush=True)
+        print(f"[{a.case}] continuing from the checkpoint at step {t_start} = {t_start/su.t_unit:.1f} H/U", flush=True)
     else:
         sim.init(noise=a.noise, seed=a.seed)
 
@@ -556,9 +556,10 @@ def main(argv=None):
             if t % 5000 == 0:
                 un = sim.macro()
                 divergence_guard(un, a.case, t)
-                chk = f" rho[{un[0].min():.3f},{un[0].max():.3f}] |u|max/U_H={np.abs(un[1:]).max()/su.UH:.2f}"
-            print(f"[{a.case}] step {t}/{n_total}  t={t/su.t_unit:.1f} H/U  {mlups:.0f} MLUPS  "
-                  f"eta {el/done*(n_total-t-1)/60:.1f} min{chk}", flush=True)
+                chk = (f"; density {un[0].min():.3f} .. {un[0].max():.3f}, largest |u| / U_H "
+                       f"{np.abs(un[1:]).max()/su.UH:.2f}")
+            print(f"[{a.case}] {t} of {n_total} steps ({t/su.t_unit:.1f} H/U), {mlups:.0f} MLUPS, "
+                  f"{el/done*(n_total-t-1)/60:.1f} min to go{chk}", flush=True)
         t += 1
 
     fh = sim.force_rows()
