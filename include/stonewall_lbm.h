@@ -146,6 +146,11 @@ int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev,
  * change in rounding only.  The two-step batch uses 16. */
 int sw_set_step2_rows(int rows);
 int sw_get_step2_rows(void);
+/* Order of the two-step kernels' tiles over the workgroups (no reference counterpart):
+ * 1 (default) gives each XCD (each with its own L2) one contiguous band of tiles, so the halo
+ * rows and columns a tile reads are read by its neighbours in the same L2; 0 numbers them
+ * plainly.  Results do not depend on it. */
+int sw_set_step2_tile_order(int banded);
 
 /* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
  * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept

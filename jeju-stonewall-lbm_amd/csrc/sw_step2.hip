@@ -4,24 +4,39 @@
 
 namespace sw {
 
-template <class S, int TY>
-__global__ __launch_bounds__(64 * TY, 1) void k_step2(StepArgs a, int ntx, int ntiles) {
-    // (Numbering the tiles so that each XCD, with its own L2, gets runs of neighbouring tiles
-    // measured no faster: the workgroup order is plain.)
-    __shared__ Step2Lds<TY> L;
-    step2_tile<S, TY, false>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, (int)blockIdx.x, L);
+// Tile of workgroup b.  Workgroups go to the XCDs round-robin (b mod NXCD), each XCD with its
+// own L2; with `banded`, XCD x takes one contiguous band of tiles (tile rows of neighbouring
+// y), so the halo rows and halo columns a tile reads are read by tiles in the same L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int step2_tile_of(int b, int ntiles, bool banded) {
+    if (!banded) return b;
+    const int x = b % NXCD, j = b / NXCD;
+    // workgroups b = x, x + NXCD, ...: XCD x holds (ntiles - x + NXCD - 1) / NXCD of them; its band
+    // starts after the bands of XCDs 0 .. x-1
+    const int q = ntiles / NXCD, r = ntiles % NXCD;
+    const int start = x * q + (x < r ? x : r);
+    return start + j;
 }
 
 template <class S, int TY>
-__global__ __launch_bounds__(64 * TY, 1) void k_step2_one(StepArgs a, int ntx, int ntiles) {
+__global__ __launch_bounds__(64 * TY, 1) void k_step2(StepArgs a, int ntx, int ntiles, int banded) {
     __shared__ Step2Lds<TY> L;
-    step2_tile<S, TY, true>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, (int)blockIdx.x, L);
+    step2_tile<S, TY, false>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles,
+                             step2_tile_of((int)blockIdx.x, ntiles, banded != 0), L);
+}
+
+template <class S, int TY>
+__global__ __launch_bounds__(64 * TY, 1) void k_step2_one(StepArgs a, int ntx, int ntiles, int banded) {
+    __shared__ Step2Lds<TY> L;
+    step2_tile<S, TY, true>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles,
+                            step2_tile_of((int)blockIdx.x, ntiles, banded != 0), L);
 }
 
 // Tile rows per workgroup (15 or 16) for a lattice: the setting of sw_set_step2_rows, or by
 // default the candidate that fills the compute units in the fewest rounds of workgroups,
 // each round weighted by its rows per output row.
 static int g_step2_rows = 0;
+static int g_step2_banded = 1;
 
 static int step2_rows(int nx, int ny, int cus) {
     if (g_step2_rows) return g_step2_rows;
@@ -34,12 +49,12 @@ static void launch_step2(const StepArgs& a, int cus, hipStream_t s) {
     const int ntx = (a.nx + 63) / 64;
     if (step2_rows(a.nx, a.ny, cus) == 15) {
         const int tiles = step2_tiles_for(a.nx, a.ny, 15);
-        if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles);
-        else hipLaunchKernelGGL((k_step2<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles);
+        if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles, g_step2_banded);
+        else hipLaunchKernelGGL((k_step2<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles, g_step2_banded);
     } else {
         const int tiles = step2_tiles_for(a.nx, a.ny, 16);
-        if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles);
-        else hipLaunchKernelGGL((k_step2<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles);
+        if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles, g_step2_banded);
+        else hipLaunchKernelGGL((k_step2<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles, g_step2_banded);
     }
 }
 
@@ -102,6 +117,12 @@ int sw_set_step2_rows(int rows) {
 }
 
 int sw_get_step2_rows(void) { return g_step2_rows; }
+
+int sw_set_step2_tile_order(int banded) {
+    SW_CHECK_ARG(banded == 0 || banded == 1, "banded must be 0 (plain) or 1 (one band of tiles per XCD)");
+    g_step2_banded = banded;
+    return SW_OK;
+}
 
 long long sw_step2_partials_per_step(int nx, int ny, int nz) {
     if (check_dims(nx, ny, nz)) return -1;
