@@ -4,18 +4,11 @@
 
 namespace sw {
 
-// Tile of workgroup b.  Workgroups go to the XCDs round-robin (b mod NXCD), each XCD with its
-// own L2; with `banded`, XCD x takes one contiguous band of tiles (tile rows of neighbouring
-// y), so the halo rows and halo columns a tile reads are read by tiles in the same L2.
-constexpr int NXCD = 8;
+// Tile of workgroup b: with `banded`, the j-th workgroup of XCD x (b = x + j NXCD) takes tile
+// band_start(x) + j (XCD x holds ceil((ntiles - x) / NXCD) workgroups, its band's length).
 __device__ __forceinline__ int step2_tile_of(int b, int ntiles, bool banded) {
     if (!banded) return b;
-    const int x = b % NXCD, j = b / NXCD;
-    // workgroups b = x, x + NXCD, ...: XCD x holds (ntiles - x + NXCD - 1) / NXCD of them; its band
-    // starts after the bands of XCDs 0 .. x-1
-    const int q = ntiles / NXCD, r = ntiles % NXCD;
-    const int start = x * q + (x < r ? x : r);
-    return start + j;
+    return band_start(b % NXCD, ntiles) + b / NXCD;
 }
 
 template <class S, int TY>

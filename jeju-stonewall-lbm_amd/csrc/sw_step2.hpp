@@ -544,6 +544,15 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
 }
 
 
+// Workgroups are handed to the XCDs round-robin (workgroup b to XCD b mod NXCD), each XCD
+// with its own L2.  The two-step launches give XCD x one contiguous band of the tile numbers,
+// [band_start(x), band_start(x + 1)), so the halo rows and columns a tile reads were mostly
+// just read by its neighbours in the same L2.
+constexpr int NXCD = 8;
+__host__ __device__ constexpr int band_start(int x, int ntiles) {
+    return x * (ntiles / NXCD) + (x < ntiles % NXCD ? x : ntiles % NXCD);
+}
+
 // Tile rows of a two-step launch: TY (15 or 16) rows per workgroup, TY - 2 of them output rows.
 __host__ __device__ constexpr int step2_tiles_for(int nx, int ny, int ty) {
     return ((nx + 63) / 64) * ((ny + ty - 3) / (ty - 2));
