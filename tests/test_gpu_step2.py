@@ -145,3 +145,23 @@ def test_two_step_headline_lattice(gpu, storage):
     assert np.allclose(sim.force_rows(), F, rtol=1e-11, atol=1e-14 * np.abs(F).max())
     del sim
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_two_step_tile_order(gpu, storage):
+    """The banded tile order (one contiguous band of tiles per XCD, the default) and the
+    plain order give the same populations and force rows; 30 tiles (not a multiple of the 8
+    XCDs), odd steps included."""
+    from stonewall3d import _native as nat
+    flag, uin = stones(330, 60, 12, 8, 60)
+    gust = gust_table(make_gust(4), 5)
+    out = []
+    try:
+        for banded in (1, 0):
+            nat.call("sw_set_step2_tile_order", banded)
+            differ, Fg, Fo = compare(flag, uin, 5, storage, 1, gust=gust)
+            assert differ == 0, f"{differ} populations differ (banded={banded})"
+            out.append(Fg)
+    finally:
+        nat.call("sw_set_step2_tile_order", 1)
+    assert np.array_equal(out[0], out[1])
