@@ -139,11 +139,13 @@ int sw_step2_one(const void* f_in, void* f_out, int dtype, const uint32_t* links
 /* force[t][z][0..1] from the sw_step2 partials ring, as sw_force_reduce does for sw_step's. */
 int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int nx, int ny, int nz,
                      double* force, void* stream);
-/* Tile rows of the two-step kernels (no reference counterpart): 0 (default) picks 15 or 16 per
- * lattice to fill the compute units in the fewest rounds; 15 or 16 fixes it for the process.
- * The rows set the partials layout (sw_step2_partials_per_step, sw_force_reduce2): keep one
- * setting for the life of a partials ring.  Populations do not depend on it; force rows
- * change in rounding only.  The two-step batch uses 16. */
+/* Tile rows of the two-step kernels (no reference counterpart): 0 (default) picks per lattice
+ * 15 or 16 rows, one workgroup per tile, or, for lattices with fewer tiles than the device has
+ * compute units, 13 rows with each tile's z sweep cut into segments (sw_set_step2_segments);
+ * 13, 15 or 16 fixes it for the process.  The rows set the partials layout
+ * (sw_step2_partials_per_step, sw_force_reduce2): keep one setting for the life of a partials
+ * ring.  Populations do not depend on it; force rows change in rounding only.  The two-step
+ * batch uses 16. */
 int sw_set_step2_rows(int rows);
 int sw_get_step2_rows(void);
 /* Order of the two-step kernels' tiles over the workgroups (no reference counterpart):
@@ -151,6 +153,14 @@ int sw_get_step2_rows(void);
  * rows and columns a tile reads are read by its neighbours in the same L2; 0 numbers them
  * plainly.  Results do not depend on it. */
 int sw_set_step2_tile_order(int banded);
+/* z segments per tile of the two-step kernels with 13-row tiles (no reference counterpart):
+ * each tile's sweep is cut into this many workgroups (1 .. 32, at most nz - 1), each
+ * recomputing the first step of the plane below its segment, so that a lattice with fewer
+ * tiles than compute units fills the device.  0 (default) picks the count per lattice.
+ * Lattices with 15- or 16-row tiles are not cut.  Populations and force rows do not depend on
+ * it.  sw_get_step2_segments returns the count a lattice gets (-1 on bad dimensions). */
+int sw_set_step2_segments(int segments);
+int sw_get_step2_segments(int nx, int ny, int nz);
 
 /* Kernel variants of sw_step (no reference counterpart: the reference has one Warp
  * kernel; these are the tile shapes the autotuner chooses between).  The choice is kept

@@ -25,27 +25,81 @@ __global__ __launch_bounds__(64 * TY, 1) void k_step2_one(StepArgs a, int ntx, i
                             step2_tile_of((int)blockIdx.x, ntiles, banded != 0), L);
 }
 
-// Tile rows per workgroup (15 or 16) for a lattice: the setting of sw_set_step2_rows, or by
-// default the candidate that fills the compute units in the fewest rounds of workgroups,
-// each round weighted by its rows per output row.
+// The same with each tile's sweep cut into nseg z segments, for lattices with fewer tiles than
+// compute units: work item i = g * ntiles + tile (segment g of `tile`, numbered through the
+// banded order), segment g holding the output planes [1 + g (nz-1) / nseg, 1 + (g+1) (nz-1)
+// / nseg); each segment recomputes the first step of the plane below it.  Launched with 13-row
+// tiles: with a run-time za (step2_tile) the kernel needs 128 VGPRs, which 13 wavefronts per
+// workgroup allow without spilling (at 15 or 16 rows it spilled about 30 and ran 1.46x slower
+// per plane).
+template <class S, int TY, bool SINGLE>
+__global__ __launch_bounds__(64 * TY, 1) void k_step2_seg(StepArgs a, int ntx, int ntiles, int nseg, int banded) {
+    __shared__ Step2Lds<TY> L;
+    const int item = step2_tile_of((int)blockIdx.x, ntiles * nseg, banded != 0);
+    const int g = item / ntiles;
+    const int za = 1 + (int)(((long long)g * (a.nz - 1)) / nseg);
+    const int zb = 1 + (int)(((long long)(g + 1) * (a.nz - 1)) / nseg);
+    step2_tile<S, TY, SINGLE>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, item - g * ntiles, L, za, zb);
+}
+
+constexpr int SEG_ROWS = 13;          // tile rows of the segmented launch
+
+// Launch shape of a lattice: tile rows TY (the partials layout: sw_step2_partials_per_step)
+// and z segments S.  TY is the setting of sw_set_step2_rows, or by default: 15 or 16 rows
+// uncut (the one of fewer rounds of workgroups, each weighted by its rows), unless SEG_ROWS
+// rows cut into S = 1 .. 32 segments are estimated 3 % faster.  The estimate: rounds of
+// workgroups over the compute units (one per unit: the LDS) times a workgroup's sweep (its
+// planes, half a plane for the recomputed first step below a segment, about two planes of
+// fill and drain) times TY / (TY - 2) rows per output row.  S is the setting of
+// sw_set_step2_segments for SEG_ROWS tiles (1 for 15 and 16), or that estimate's.  The
+// settings never change TY's default, so a lattice keeps its layout.
 static int g_step2_rows = 0;
 static int g_step2_banded = 1;
+static int g_step2_segments = 0;
 
-static int step2_rows(int nx, int ny, int cus) {
+static int step2_max_segments(int nz) { return nz - 1 < 32 ? (nz - 1 < 1 ? 1 : nz - 1) : 32; }
+
+static double step2_cost(int nx, int ny, int nz, int cus, int ty, int sg) {
+    const long long items = (long long)step2_tiles_for(nx, ny, ty) * sg;
+    const double rounds = (double)((items + cus - 1) / cus);
+    return rounds * ((double)(nz - 1) / sg + (sg > 1 ? 0.5 : 0.0) + 2.0) * ((double)ty / (ty - 2));
+}
+
+static int step2_auto_segments(int nx, int ny, int nz, int cus) {
+    int best = 1;
+    for (int sg = 2; sg <= step2_max_segments(nz); sg++)
+        if (step2_cost(nx, ny, nz, cus, SEG_ROWS, sg) < step2_cost(nx, ny, nz, cus, SEG_ROWS, best) - 1e-9) best = sg;
+    return best;
+}
+
+static int step2_rows(int nx, int ny, int nz, int cus) {
     if (g_step2_rows) return g_step2_rows;
-    auto cost = [&](int ty) { return ((step2_tiles_for(nx, ny, ty) + cus - 1) / cus) * ty; };
-    return cost(15) < cost(16) ? 15 : 16;
+    // between the uncut shapes: the fewer rounds of workgroups, each weighted by its rows
+    const int r15 = (step2_tiles_for(nx, ny, 15) + cus - 1) / cus, r16 = (step2_tiles_for(nx, ny, 16) + cus - 1) / cus;
+    const int uncut = r15 * 15 < r16 * 16 ? 15 : 16;
+    const double cseg = step2_cost(nx, ny, nz, cus, SEG_ROWS, step2_auto_segments(nx, ny, nz, cus));
+    return cseg < 0.97 * step2_cost(nx, ny, nz, cus, uncut, 1) ? SEG_ROWS : uncut;
+}
+
+static int step2_segments(int nx, int ny, int nz, int cus, int ty) {
+    if (ty != SEG_ROWS) return 1;
+    if (g_step2_segments) return g_step2_segments < step2_max_segments(nz) ? g_step2_segments : step2_max_segments(nz);
+    return step2_auto_segments(nx, ny, nz, cus);
 }
 
 template <class S, bool SINGLE>
 static void launch_step2(const StepArgs& a, int cus, hipStream_t s) {
     const int ntx = (a.nx + 63) / 64;
-    if (step2_rows(a.nx, a.ny, cus) == 15) {
-        const int tiles = step2_tiles_for(a.nx, a.ny, 15);
+    const int ty = step2_rows(a.nx, a.ny, a.nz, cus);
+    const int tiles = step2_tiles_for(a.nx, a.ny, ty);
+    if (ty == SEG_ROWS) {
+        const int nseg = step2_segments(a.nx, a.ny, a.nz, cus, ty);
+        hipLaunchKernelGGL((k_step2_seg<S, SEG_ROWS, SINGLE>), dim3((unsigned)(tiles * nseg)), dim3(64 * SEG_ROWS), 0,
+                           s, a, ntx, tiles, nseg, g_step2_banded);
+    } else if (ty == 15) {
         if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles, g_step2_banded);
         else hipLaunchKernelGGL((k_step2<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles, g_step2_banded);
     } else {
-        const int tiles = step2_tiles_for(a.nx, a.ny, 16);
         if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles, g_step2_banded);
         else hipLaunchKernelGGL((k_step2<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles, g_step2_banded);
     }
@@ -104,12 +158,25 @@ using namespace sw;
 extern "C" {
 
 int sw_set_step2_rows(int rows) {
-    SW_CHECK_ARG(rows == 0 || rows == 15 || rows == 16, "rows must be 0 (automatic), 15 or 16");
+    SW_CHECK_ARG(rows == 0 || rows == SEG_ROWS || rows == 15 || rows == 16, "rows must be 0 (automatic), 13, 15 or 16");
     g_step2_rows = rows;
     return SW_OK;
 }
 
 int sw_get_step2_rows(void) { return g_step2_rows; }
+
+int sw_set_step2_segments(int segments) {
+    SW_CHECK_ARG(segments >= 0 && segments <= 32, "segments must be 0 (automatic) .. 32");
+    g_step2_segments = segments;
+    return SW_OK;
+}
+
+int sw_get_step2_segments(int nx, int ny, int nz) {
+    if (check_dims(nx, ny, nz)) return -1;
+    int cus = 0;
+    if (device_cus(&cus)) return -1;
+    return step2_segments(nx, ny, nz, cus, step2_rows(nx, ny, nz, cus));
+}
 
 int sw_set_step2_tile_order(int banded) {
     SW_CHECK_ARG(banded == 0 || banded == 1, "banded must be 0 (plain) or 1 (one band of tiles per XCD)");
@@ -121,7 +188,7 @@ long long sw_step2_partials_per_step(int nx, int ny, int nz) {
     if (check_dims(nx, ny, nz)) return -1;
     int cus = 0;
     if (device_cus(&cus)) return -1;
-    return (long long)nz * step2_tiles_for(nx, ny, step2_rows(nx, ny, cus)) * 2;
+    return (long long)nz * step2_tiles_for(nx, ny, step2_rows(nx, ny, nz, cus)) * 2;
 }
 
 int sw_step2(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
@@ -146,7 +213,7 @@ int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev,
     SW_CHECK_ARG(t0 >= 0, "t0 must be non-negative");
     int cus = 0;
     SW_CHECK_RC(device_cus(&cus));
-    const int P = step2_tiles_for(nx, ny, step2_rows(nx, ny, cus));
+    const int P = step2_tiles_for(nx, ny, step2_rows(nx, ny, nz, cus));
     hipLaunchKernelGGL(k_force_reduce2, dim3(nz, nsteps), dim3(64), 0, (hipStream_t)stream, partials, ring, t0, t_dev,
                        nz, P, force);
     SW_HIP(hipGetLastError());

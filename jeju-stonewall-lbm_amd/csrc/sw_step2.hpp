@@ -60,8 +60,15 @@ __device__ __forceinline__ float& lds_row(Step2Lds<TY>& L, int j, int plane, int
 // of pairs, one step with the same tile decomposition and force-partials layout (SINGLE =
 // true: the output rows compute the first step of every plane and store it; the halo rows
 // idle).  P: tiles per z plane of the partials layout; L: the workgroup's LDS.
+// [za, zb): the z planes whose outputs (and force partials) this call writes, a segment of
+// the tile's sweep (za = 1: from the plane above the ground); the two-step sweep then starts
+// at plane za - 1, recomputing the first step of the plane below the segment, so a tile cut
+// into segments gives the populations and partials of one uncut sweep.  (The defaults, the
+// whole sweep, are constants the compiler folds: with a run-time za the FP32 kernel needs
+// 35 more VGPRs than it has, so the segmented launch is a separate instantiation.)
 template <class S, int TY, bool SINGLE, class A>
-__device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int tile, Step2Lds<TY>& L) {
+__device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int tile, Step2Lds<TY>& L, int za = 1,
+                                           int zb = 1 << 30) {
     using Cd = Codec<S>;
     using T = typename Cd::T;
     using B = typename Cd::B;
@@ -85,6 +92,8 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
     const T smag = ((T(18) * (T)sqrt(T(2))) * cs) * cs;
     const uint32_t cw = chunk_words(nx);
     const uint32_t* cflags = a.links + N;
+    za = za < 1 ? 1 : za;
+    zb = zb > nz ? nz : zb;
 
     // ---- this wavefront's row
     const int yg = ty * OY - 1 + w;                   // may be -1 or >= ny
@@ -306,11 +315,11 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
             f2x += L.red[s][r][2]; f2z += L.red[s][r][3];
         }
         const int k1 = slot_plane, k2 = slot_plane - 1;
-        if (k1 >= 1 && k1 < nz) {
+        if (k1 >= za && k1 < zb) {
             double* p = a.partials + (((size_t)(t0 % a.ring) * nz + k1) * P + tile) * 2;
             p[0] = f1x; p[1] = f1z;
         }
-        if (k2 >= 1 && k2 < nz) {
+        if (k2 >= za && k2 < zb) {
             double* p = a.partials + (((size_t)((t0 + 1) % a.ring) * nz + k2) * P + tile) * 2;
             p[0] = f2x; p[1] = f2z;
         }
@@ -325,19 +334,19 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         // ---- one step: the output rows compute f1 of every plane and store it; the force of
         // step t0 goes to slot t0 % ring in the per-tile layout, summed as the pair's first step
         auto flush1 = [&](int kf) {
-            if (!force || lane != 0 || kf < 1 || kf >= nz) return;
+            if (!force || lane != 0 || kf < za || kf >= zb) return;
             const int s = kf % 3;
             double fx = 0.0, fz = 0.0;
             for (int r = 1; r <= OY; r++) { fx += L.red[s][r][0]; fz += L.red[s][r][1]; }
             double* p = a.partials + (((size_t)(t0 % a.ring) * nz + kf) * P + tile) * 2;
             p[0] = fx; p[1] = fz;
         };
-        if (force && w == 0 && lane == 0) {
+        if (force && w == 0 && lane == 0 && za == 1) {
             double* p = a.partials + (((size_t)(t0 % a.ring) * nz + 0) * P + tile) * 2;
             p[0] = 0.0; p[1] = 0.0;
         }
 #pragma clang loop unroll(disable)
-        for (int k = 1; k < nz; k++) {
+        for (int k = za; k < zb; k++) {
             if (out_row) {
                 T fl[Q];
                 T fxw = T(0), fzw = T(0);
@@ -379,28 +388,31 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
                 L.red[k % 3][w][1] = 0.0;
             }
             __syncthreads();
-            if (w == 0 && k >= 2) flush1(k - 1);
+            if (w == 0 && k > za) flush1(k - 1);
         }
         __syncthreads();
-        if (w == 0) flush1(nz - 1);
+        if (w == 0) flush1(zb - 1);
         return;
     }
 
-    // ---- prologue: the ground plane's partials are zero; halo and loads of plane 1
-    if (force && w == 0 && lane == 0) {
+    // ---- prologue: the ground plane's partials are zero; halo of the first plane swept
+    // (ks: plane 1, or the plane below the segment, whose first step the second step needs)
+    const int ks = za > 1 ? za - 1 : 1;
+    const int ke = zb >= nz ? nz : zb;                     // last iteration (plane ke's first step)
+    if (force && w == 0 && lane == 0 && za == 1) {
         for (int s = 0; s < 2; s++) {
             double* p = a.partials + (((size_t)((t0 + s) % a.ring) * nz + 0) * P + tile) * 2;
             p[0] = 0.0; p[1] = 0.0;
         }
     }
-    if (halo_wave && nz > 1) halo_f1(1);
+    if (halo_wave && ks < nz) halo_f1(ks);
 
     T h0[3] = {}, hp1[3] = {}, hp2[3] = {}, hm1[3] = {};   // own-column f1 populations kept in registers
     T c6 = T(0), c12 = T(0), c13 = T(0);                     // f1(k) of populations 6, 12, 13
     uint32_t m_prev = 0;                                     // link mask of this lane's cell at plane k-1
 
 #pragma clang loop unroll(disable)
-    for (int k = 1; k <= nz; k++) {
+    for (int k = ks; k <= ke; k++) {
         // ================= phase 1: f1 at plane k
         T pk[Q];                                  // what the second step reads of f1(k)
         uint32_t m_cur = 0;
@@ -439,7 +451,7 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         // ================= phase 2: f2 at plane kk = k-1 (output rows); halo f1 at k+1 (halo rows)
         const int kk = k - 1;
         float fbx = 0.0f, fbz = 0.0f;
-        if (out_row && kk >= 1) {
+        if (out_row && kk >= za) {
             const bool top = kk == nz - 1;
             T g[Q];
             auto shr = [&](T v, T edge) { return __uint_as_float(wave_shr1(__float_as_uint(v), __float_as_uint(edge))); };
@@ -528,8 +540,8 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
                 if (st_ok) { fbx = (float)fxw; fbz = (float)fzw; }
             }
         }
-        if (halo_wave && k + 1 < nz) halo_f1(k + 1);
-        if (w == 0 && k >= 2) flush(k - 1);
+        if (halo_wave && k + 1 < nz && k < ke) halo_f1(k + 1);
+        if (w == 0 && k > ks) flush(k - 1);
         // per-wavefront force sums of this iteration
         if (force) wave_force(fbx, fbz, k % 3, 2);
         // rotate the registers: plane k becomes k-1
@@ -540,7 +552,7 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         m_prev = m_cur;
     }
     __syncthreads();
-    if (w == 0) flush(nz);
+    if (w == 0) flush(ke);
 }
 
 

@@ -41,6 +41,8 @@ SIGNATURES = {
     "sw_set_step2_rows": (_c_int, [_c_int]),
     "sw_get_step2_rows": (_c_int, []),
     "sw_set_step2_tile_order": (_c_int, [_c_int]),
+    "sw_set_step2_segments": (_c_int, [_c_int]),
+    "sw_get_step2_segments": (_c_int, [_c_int, _c_int, _c_int]),
     "sw_step_variants": (_c_int, []),
     "sw_set_step_variant": (_c_int, [_c_int, _c_int]),
     "sw_get_step_variant": (_c_int, [_c_int]),

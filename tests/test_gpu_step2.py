@@ -165,3 +165,80 @@ def test_two_step_tile_order(gpu, storage):
     finally:
         nat.call("sw_set_step2_tile_order", 1)
     assert np.array_equal(out[0], out[1])
+
+
+class step2_segments:
+    """Fixes the two-step kernels' z segments per tile for a `with` block (0: automatic)."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __enter__(self):
+        from stonewall3d import _native as nat
+        nat.call("sw_set_step2_segments", self.n)
+
+    def __exit__(self, *exc):
+        from stonewall3d import _native as nat
+        nat.call("sw_set_step2_segments", 0)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("ybc", [0, 1])
+def test_two_step_z_segments(gpu, storage, ybc):
+    """Tiles cut into z segments (each recomputes the first step of the plane below it):
+    1 (the uncut kernel), 2, 3, 5 and 16 segments of a 17-plane sweep (down to one plane per
+    segment, the last ending at the free top), 7 steps (the single-step launch cut the same
+    way).  Populations bit-identical to the oracle; force rows bit-identical to the uncut
+    sweep."""
+    from stonewall3d import _native as nat
+    flag, uin = stones(200, 31, 18, 12, 60)
+    assert nat.lib().sw_get_step2_segments(200, 31, 18) > 1       # 13-row tiles, cut by default
+    gust = gust_table(make_gust(8), 7)
+    ref = None
+    for n in (1, 2, 3, 5, 16):
+        with step2_segments(n):
+            differ, Fg, Fo = compare(flag, uin, 7, storage, ybc, gust=gust)
+        assert differ == 0, f"{differ} populations differ with {n} segments"
+        if ref is None:
+            ref = Fg
+            assert np.allclose(Fg, Fo, rtol=1e-12, atol=1e-15 * max(np.abs(Fo).max(), 1e-300))
+        assert np.array_equal(Fg, ref), f"force rows with {n} segments differ from the uncut sweep"
+
+
+@pytest.mark.parametrize("rows", [15, 16])
+@pytest.mark.parametrize("dims,X0", [((128, 29, 14), 60), ((257, 17, 9), 190), ((130, 3, 6), 64)])
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_two_step_uncut_small_lattices(gpu, rows, dims, X0, storage):
+    """The uncut kernel (15 or 16 rows, one workgroup per tile) on lattices that by default
+    take 13-row tiles cut into segments (fewer tiles than compute units): partial x and y
+    tiles, both side boundaries."""
+    from stonewall3d import _native as nat
+    nx, ny, nz = dims
+    flag, uin = stones(nx, ny, nz, max(2, nz - 4), X0)
+    nat.call("sw_set_step2_rows", rows)
+    try:
+        for ybc in (0, 1):
+            check(flag, uin, 6, storage, ybc, gust=gust_table(make_gust(2), 6))
+    finally:
+        nat.call("sw_set_step2_rows", 0)
+
+
+def test_two_step_reference_lattice_segments(gpu):
+    """The reference's default lattice (800x160x321, 169 tiles: cut into segments by default)
+    gives the populations and force rows of the uncut sweep after 3 steps (FP32)."""
+    from stonewall3d import _native as nat
+    w = workloads.reference(H=40, porosity=0.2, seed=7, n_steps=3)
+    assert nat.lib().sw_get_step2_segments(800, 160, 321) > 1
+    out = []
+    for n in (0, 1):
+        with step2_segments(n):
+            sim = Simulation(w.flag, w.uin, dtype="f32", two_step=True, ring=4, **w.sim_kwargs())
+            sim.init(noise=0.02, seed=0)
+            sim.step(3)
+            out.append((sim.f[0].clone(), sim.force_rows()))
+            del sim
+    fl = torch.from_numpy(w.flag == 0).cuda().permute(2, 1, 0).reshape(-1)
+    assert torch.equal(out[0][0].view(19, -1)[:, fl], out[1][0].view(19, -1)[:, fl])
+    assert np.array_equal(out[0][1], out[1][1])
+    del out
+    torch.cuda.empty_cache()
