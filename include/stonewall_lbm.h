@@ -141,11 +141,12 @@ int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev,
                      double* force, void* stream);
 /* Tile rows of the two-step kernels (no reference counterpart): 0 (default) picks per lattice
  * 15 or 16 rows, one workgroup per tile, or, for lattices with fewer tiles than the device has
- * compute units, 13 rows with each tile's z sweep cut into segments (sw_set_step2_segments);
- * 13, 15 or 16 fixes it for the process.  The rows set the partials layout
+ * compute units, 12 or 13 rows with each tile's z sweep cut into segments
+ * (sw_set_step2_segments; the 12-row kernel keeps the next plane's loads in flight across each
+ * plane's barrier); 12, 13, 15 or 16 fixes it for the process.  The rows set the partials layout
  * (sw_step2_partials_per_step, sw_force_reduce2): keep one setting for the life of a partials
  * ring.  Populations do not depend on it; force rows change in rounding only.  The two-step
- * batch uses 16. */
+ * batch uses sw_batch_step2_rows(). */
 int sw_set_step2_rows(int rows);
 int sw_get_step2_rows(void);
 /* Order of the two-step kernels' tiles over the workgroups (no reference counterpart):
@@ -153,7 +154,7 @@ int sw_get_step2_rows(void);
  * rows and columns a tile reads are read by its neighbours in the same L2; 0 numbers them
  * plainly.  Results do not depend on it. */
 int sw_set_step2_tile_order(int banded);
-/* z segments per tile of the two-step kernels with 13-row tiles (no reference counterpart):
+/* z segments per tile of the two-step kernels with 12- or 13-row tiles (no reference counterpart):
  * each tile's sweep is cut into this many workgroups (1 .. 32, at most nz - 1), each
  * recomputing the first step of the plane below its segment, so that a lattice with fewer
  * tiles than compute units fills the device.  0 (default) picks the count per lattice.
@@ -290,15 +291,17 @@ int sw_batch_graph_create(void* table, int max_instances, int dtype, int K, int 
  * or BF16 (one dtype per table) with population arrays below 4 GiB.  Partials ring per
  * instance: ring * sw_batch_partials_per_step(nx, ny, nz) doubles (enough for either form).
  * sw_batch_step2(p): steps t_i + 2p and t_i + 2p + 1 of every instance with two or more left
- * at t_i + 2p, persistent `blocks` workgroups (one per compute unit fits) taking 64 x 14-cell
- * tiles from a counter in the table header; the buffers swap once per launch that touches
+ * at t_i + 2p, persistent `blocks` workgroups (one per compute unit fits) taking 64 x 10-cell
+ * tiles (sw_batch_step2_rows() rows with their two halo rows) from a counter in the table header; the buffers swap once per launch that touches
  * an instance.  sw_batch_force_reduce2: force rows of steps t_i .. t_i + K - 1 (ring >= K).
  * sw_batch_advance2: t_i += K (K even, clamped to t_end) and the parity of the launches.
  * sw_batch_graph_create2: K / 2 pair launches + the tail launch + force reduction + advance
  * as one hipGraph.
  * Populations are bit-identical to single steps; force rows to the single-lattice run with
- * sw_set_step2_rows(16) (pairs by sw_step2, an odd last step by sw_step2_one). */
+ * sw_set_step2_rows(sw_batch_step2_rows()) (pairs by sw_step2, an odd last step by sw_step2_one). */
 long long sw_batch_partials_per_step(int nx, int ny, int nz);
+/* Tile rows of the two-step batch's tiles, halo rows included (12). */
+int sw_batch_step2_rows(void);
 int sw_batch_step2(void* table, int dtype, int p, int blocks, void* stream);
 /* After the K / 2 pair launches of a chunk: the odd last step of every instance with fewer
  * than K (and an odd number of) steps left at the chunk's start. */

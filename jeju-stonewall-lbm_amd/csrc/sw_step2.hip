@@ -11,11 +11,11 @@ __device__ __forceinline__ int step2_tile_of(int b, int ntiles, bool banded) {
     return band_start(b % NXCD, ntiles) + b / NXCD;
 }
 
-template <class S, int TY>
+template <class S, int TY, bool PF = false>
 __global__ __launch_bounds__(64 * TY, 1) void k_step2(StepArgs a, int ntx, int ntiles, int banded) {
     __shared__ Step2Lds<TY> L;
-    step2_tile<S, TY, false>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles,
-                             step2_tile_of((int)blockIdx.x, ntiles, banded != 0), L);
+    step2_tile<S, TY, false, StepArgs, PF>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles,
+                                           step2_tile_of((int)blockIdx.x, ntiles, banded != 0), L);
 }
 
 template <class S, int TY>
@@ -32,27 +32,35 @@ __global__ __launch_bounds__(64 * TY, 1) void k_step2_one(StepArgs a, int ntx, i
 // tiles: with a run-time za (step2_tile) the kernel needs 128 VGPRs, which 13 wavefronts per
 // workgroup allow without spilling (at 15 or 16 rows it spilled about 30 and ran 1.46x slower
 // per plane).
-template <class S, int TY, bool SINGLE>
+template <class S, int TY, bool SINGLE, bool PF = false>
 __global__ __launch_bounds__(64 * TY, 1) void k_step2_seg(StepArgs a, int ntx, int ntiles, int nseg, int banded) {
     __shared__ Step2Lds<TY> L;
     const int item = step2_tile_of((int)blockIdx.x, ntiles * nseg, banded != 0);
     const int g = item / ntiles;
     const int za = 1 + (int)(((long long)g * (a.nz - 1)) / nseg);
     const int zb = 1 + (int)(((long long)(g + 1) * (a.nz - 1)) / nseg);
-    step2_tile<S, TY, SINGLE>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, item - g * ntiles, L, za, zb);
+    step2_tile<S, TY, SINGLE, StepArgs, PF>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, item - g * ntiles, L,
+                                            za, zb);
 }
 
 constexpr int SEG_ROWS = 13;          // tile rows of the segmented launch
+constexpr int PF_ROWS = 12;           // tile rows of the launch with next-plane loads in flight (PF)
 
 // Launch shape of a lattice: tile rows TY (the partials layout: sw_step2_partials_per_step)
 // and z segments S.  TY is the setting of sw_set_step2_rows, or by default: 15 or 16 rows
 // uncut (the one of fewer rounds of workgroups, each weighted by its rows), unless SEG_ROWS
-// rows cut into S = 1 .. 32 segments are estimated 3 % faster.  The estimate: rounds of
-// workgroups over the compute units (one per unit: the LDS) times a workgroup's sweep (its
-// planes, half a plane for the recomputed first step below a segment, about two planes of
-// fill and drain) times TY / (TY - 2) rows per output row.  S is the setting of
-// sw_set_step2_segments for SEG_ROWS tiles (1 for 15 and 16), or that estimate's.  The
-// settings never change TY's default, so a lattice keeps its layout.
+// rows cut into S = 1 .. 32 segments are estimated 3 % faster; in that case (lattices whose
+// tiles leave compute units idle) PF_ROWS rows with the next plane's loads in flight, cut
+// into their own best S, where that estimate is lower.  The estimate: rounds of workgroups
+// over the compute units (one per unit: the LDS) times a workgroup's sweep (its planes, half
+// a plane for the recomputed first step below a segment, about two planes of fill and
+// drain) times TY / (TY - 2) rows per output row, times PF_COST for PF_ROWS tiles (the
+// 800x160x321 lattice's 13-row and 12-row launches measured against it).  S is the setting
+// of sw_set_step2_segments for SEG_ROWS and PF_ROWS tiles (1 for 15 and 16), or the
+// estimate's.  The settings never change TY's default, so a lattice keeps its layout.
+// (On lattices whose 15-row tiles fill the compute units in one round, e.g. 640x320x160,
+// every 12-row shape measured slower than the uncut 15 rows.)
+constexpr double PF_COST = 0.8;
 static int g_step2_rows = 0;
 static int g_step2_banded = 1;
 static int g_step2_segments = 0;
@@ -62,13 +70,14 @@ static int step2_max_segments(int nz) { return nz - 1 < 32 ? (nz - 1 < 1 ? 1 : n
 static double step2_cost(int nx, int ny, int nz, int cus, int ty, int sg) {
     const long long items = (long long)step2_tiles_for(nx, ny, ty) * sg;
     const double rounds = (double)((items + cus - 1) / cus);
-    return rounds * ((double)(nz - 1) / sg + (sg > 1 ? 0.5 : 0.0) + 2.0) * ((double)ty / (ty - 2));
+    return rounds * ((double)(nz - 1) / sg + (sg > 1 ? 0.5 : 0.0) + 2.0) * ((double)ty / (ty - 2)) *
+           (ty == PF_ROWS ? PF_COST : 1.0);
 }
 
-static int step2_auto_segments(int nx, int ny, int nz, int cus) {
+static int step2_auto_segments(int nx, int ny, int nz, int cus, int ty) {
     int best = 1;
     for (int sg = 2; sg <= step2_max_segments(nz); sg++)
-        if (step2_cost(nx, ny, nz, cus, SEG_ROWS, sg) < step2_cost(nx, ny, nz, cus, SEG_ROWS, best) - 1e-9) best = sg;
+        if (step2_cost(nx, ny, nz, cus, ty, sg) < step2_cost(nx, ny, nz, cus, ty, best) - 1e-9) best = sg;
     return best;
 }
 
@@ -77,14 +86,16 @@ static int step2_rows(int nx, int ny, int nz, int cus) {
     // between the uncut shapes: the fewer rounds of workgroups, each weighted by its rows
     const int r15 = (step2_tiles_for(nx, ny, 15) + cus - 1) / cus, r16 = (step2_tiles_for(nx, ny, 16) + cus - 1) / cus;
     const int uncut = r15 * 15 < r16 * 16 ? 15 : 16;
-    const double cseg = step2_cost(nx, ny, nz, cus, SEG_ROWS, step2_auto_segments(nx, ny, nz, cus));
-    return cseg < 0.97 * step2_cost(nx, ny, nz, cus, uncut, 1) ? SEG_ROWS : uncut;
+    const double cseg = step2_cost(nx, ny, nz, cus, SEG_ROWS, step2_auto_segments(nx, ny, nz, cus, SEG_ROWS));
+    if (!(cseg < 0.97 * step2_cost(nx, ny, nz, cus, uncut, 1))) return uncut;
+    const double cpf = step2_cost(nx, ny, nz, cus, PF_ROWS, step2_auto_segments(nx, ny, nz, cus, PF_ROWS));
+    return cpf < cseg ? PF_ROWS : SEG_ROWS;
 }
 
 static int step2_segments(int nx, int ny, int nz, int cus, int ty) {
-    if (ty != SEG_ROWS) return 1;
+    if (ty != SEG_ROWS && ty != PF_ROWS) return 1;
     if (g_step2_segments) return g_step2_segments < step2_max_segments(nz) ? g_step2_segments : step2_max_segments(nz);
-    return step2_auto_segments(nx, ny, nz, cus);
+    return step2_auto_segments(nx, ny, nz, cus, ty);
 }
 
 template <class S, bool SINGLE>
@@ -92,7 +103,19 @@ static void launch_step2(const StepArgs& a, int cus, hipStream_t s) {
     const int ntx = (a.nx + 63) / 64;
     const int ty = step2_rows(a.nx, a.ny, a.nz, cus);
     const int tiles = step2_tiles_for(a.nx, a.ny, ty);
-    if (ty == SEG_ROWS) {
+    if (ty == PF_ROWS) {
+        const int nseg = step2_segments(a.nx, a.ny, a.nz, cus, ty);
+        if (nseg > 1) {
+            hipLaunchKernelGGL((k_step2_seg<S, PF_ROWS, SINGLE, !SINGLE>), dim3((unsigned)(tiles * nseg)),
+                               dim3(64 * PF_ROWS), 0, s, a, ntx, tiles, nseg, g_step2_banded);
+        } else if constexpr (SINGLE) {
+            hipLaunchKernelGGL((k_step2_one<S, PF_ROWS>), dim3(tiles), dim3(64 * PF_ROWS), 0, s, a, ntx, tiles,
+                               g_step2_banded);
+        } else {
+            hipLaunchKernelGGL((k_step2<S, PF_ROWS, true>), dim3(tiles), dim3(64 * PF_ROWS), 0, s, a, ntx, tiles,
+                               g_step2_banded);
+        }
+    } else if (ty == SEG_ROWS) {
         const int nseg = step2_segments(a.nx, a.ny, a.nz, cus, ty);
         hipLaunchKernelGGL((k_step2_seg<S, SEG_ROWS, SINGLE>), dim3((unsigned)(tiles * nseg)), dim3(64 * SEG_ROWS), 0,
                            s, a, ntx, tiles, nseg, g_step2_banded);
@@ -158,7 +181,8 @@ using namespace sw;
 extern "C" {
 
 int sw_set_step2_rows(int rows) {
-    SW_CHECK_ARG(rows == 0 || rows == SEG_ROWS || rows == 15 || rows == 16, "rows must be 0 (automatic), 13, 15 or 16");
+    SW_CHECK_ARG(rows == 0 || rows == PF_ROWS || rows == SEG_ROWS || rows == 15 || rows == 16,
+                 "rows must be 0 (automatic), 12, 13, 15 or 16");
     g_step2_rows = rows;
     return SW_OK;
 }

@@ -66,7 +66,11 @@ __device__ __forceinline__ float& lds_row(Step2Lds<TY>& L, int j, int plane, int
 // into segments gives the populations and partials of one uncut sweep.  (The defaults, the
 // whole sweep, are constants the compiler folds: with a run-time za the FP32 kernel needs
 // 35 more VGPRs than it has, so the segmented launch is a separate instantiation.)
-template <class S, int TY, bool SINGLE, class A>
+// PF: the streamed loads of plane k+1's first step are issued at the end of phase 1 of plane
+// k, before the barrier, so they are in flight during the barrier and the second step (19
+// more registers held across phase 2: with fewer than 13 rows the workgroup runs at most 3
+// wavefronts per SIMD, which leaves 168 VGPRs for them).
+template <class S, int TY, bool SINGLE, class A, bool PF = false>
 __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int tile, Step2Lds<TY>& L, int za = 1,
                                            int zb = 1 << 30) {
     using Cd = Codec<S>;
@@ -201,12 +205,20 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
     };
 
     // ---- first step, fast path loads of plane k for this wavefront's row
-    auto f1_load = [&](int k, T (&fl)[Q]) {
+    auto f1_issue = [&](int k, B (&raw)[Q]) {
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             const uint32_t so = row_off(I, k) - (CX[i] > 0 ? ES : 0u);
-            fl[i] = Cd::decode(buf_load<B>(rin, CX[i] < 0 ? xpo : xo, so), dsh(I, k), inv_scale);
+            raw[i] = buf_load<B>(rin, CX[i] < 0 ? xpo : xo, so);
         });
+    };
+    auto f1_decode = [&](int k, const B (&raw)[Q], T (&fl)[Q]) {
+        static_for<0, Q>([&](auto I) { fl[I] = Cd::decode(raw[I], dsh(I, k), inv_scale); });
+    };
+    auto f1_load = [&](int k, T (&fl)[Q]) {
+        B raw[Q];
+        f1_issue(k, raw);
+        f1_decode(k, raw, fl);
     };
     // ---- first step near solids: the streamed values are loaded as on the fast path; on
     // links whose source is solid (mask bit set) the cell's own opposite population replaces
@@ -406,6 +418,10 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         }
     }
     if (halo_wave && ks < nz) halo_f1(ks);
+    B pre[Q];                                                // PF: streamed loads of the next plane
+    if constexpr (PF) {
+        if (need && ks < nz) f1_issue(ks, pre);
+    }
 
     T h0[3] = {}, hp1[3] = {}, hp2[3] = {}, hm1[3] = {};   // own-column f1 populations kept in registers
     T c6 = T(0), c12 = T(0), c13 = T(0);                     // f1(k) of populations 6, 12, 13
@@ -422,7 +438,8 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
             bool live = true;
             const bool fast = chunk_fast(y, k, tx);
             if (!fast) m_cur = a.links[((uint32_t)k * ny + y) * nx + xl];
-            f1_load(k, fl);
+            if constexpr (PF) f1_decode(k, pre, fl);
+            else f1_load(k, fl);
             if (!fast) {
                 live = !(m_cur & SOLID_BIT);
                 bounce_fixups(k, (((uint32_t)k * ny + y) * nx + xl) * ES, m_cur, fl, fxw, fzw);
@@ -446,6 +463,9 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
             if (force && lane == 0) { L.red[k % 3][w][0] = 0.0; L.red[k % 3][w][1] = 0.0; }
         }
         c6 = pk[6]; c12 = pk[12]; c13 = pk[13];
+        if constexpr (PF) {
+            if (need && k + 1 < nz && k + 1 <= ke) f1_issue(k + 1, pre);
+        }
         __syncthreads();
 
         // ================= phase 2: f2 at plane kk = k-1 (output rows); halo f1 at k+1 (halo rows)

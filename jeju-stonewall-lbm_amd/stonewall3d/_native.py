@@ -63,6 +63,7 @@ SIGNATURES = {
     "sw_batch_advance": (_c_int, [_p, _c_int, _c_int, _p]),
     "sw_batch_graph_create": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _p]),
     "sw_batch_partials_per_step": (_c_ll, [_c_int, _c_int, _c_int]),
+    "sw_batch_step2_rows": (_c_int, []),
     "sw_batch_step2": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
     "sw_batch_step2_tail": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
     "sw_batch_force_reduce2": (_c_int, [_p, _c_int, _c_int, _p]),

@@ -12,11 +12,11 @@ Each lattice's arithmetic is that of the single-lattice step, so its populations
 force rows are bit-identical alone and in any mix (given the same tile variant).
 
 FP32 batches run the two-step form by default: each launch advances every lattice by two
-steps (sw_batch_step2: 64 x 14-cell tiles of all lattices taken by persistent workgroups
+steps (sw_batch_step2: 64 x 10-cell tiles of all lattices taken by persistent workgroups
 from a device counter), a tail launch takes the odd last step of lattices that end inside
 the chunk, and the force rows come from the two-step per-tile partials.  A lattice's
 populations are bit-identical to single steps; its force rows are bit-identical to a
-Simulation(two_step=True) run with sw_set_step2_rows(16).
+Simulation(two_step=True) run with sw_set_step2_rows(sw_batch_step2_rows()).
 """
 import ctypes
 

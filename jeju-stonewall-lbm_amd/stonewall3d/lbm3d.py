@@ -92,7 +92,7 @@ class Simulation:
                 FP32 / FP16 / BF16 storage, population arrays below 4 GiB) and odd steps with
                 its one-step form (sw_step2_one); populations are bit-identical to single
                 steps, force rows equal to rounding (to the two-step batch bit for bit with
-                sw_set_step2_rows(16)).
+                sw_set_step2_rows(sw_batch_step2_rows())).
     """
 
     def __init__(self, flag, uin, *, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, ybc="periodic", dtype="f32",

@@ -73,9 +73,7 @@ def lattice_bytes(H, dtype, steps, ring):
     N = nx * ny * nz
     e = nat.DTYPE_BYTES[nat.DTYPES[dtype]]
     links = N + ny * nz * ((nx + 2047) // 2048)
-    nyb = (ny + 3) // 4
-    tiles = ((nx + 63) // 64) * ((ny + 13) // 14)         # the two-step batch's 64 x 14-cell tiles
-    partials = 8 * ring * nz * max(nyb, tiles) * 2
+    partials = 8 * ring * nat.lib().sw_batch_partials_per_step(nx, ny, nz)   # either step form's layout
     return 2 * 19 * N * e + 4 * links + partials + 16 * steps * nz + 4 * (nz + steps)
 
 

@@ -40,9 +40,9 @@ def alone(w, dtype, variant, seed, shift=None):
 
 def alone2(w, dtype, seed, shift=None):
     """The lattice stepped alone with the two-step kernels on the two-step batch's tiles
-    (16 rows): pairs by sw_step2, an odd last step by sw_step2_one."""
+    (sw_batch_step2_rows()): pairs by sw_step2, an odd last step by sw_step2_one."""
     old = nat.lib().sw_get_step2_rows()
-    nat.call("sw_set_step2_rows", 16)
+    nat.call("sw_set_step2_rows", nat.lib().sw_batch_step2_rows())
     try:
         sim = Simulation(w.flag, w.uin, dtype=dtype, shift=shift, two_step=True, **w.sim_kwargs())
         sim.init(noise=0.02, seed=seed)

@@ -182,27 +182,33 @@ class step2_segments:
         nat.call("sw_set_step2_segments", 0)
 
 
+@pytest.mark.parametrize("rows", [12, 13])
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 @pytest.mark.parametrize("ybc", [0, 1])
-def test_two_step_z_segments(gpu, storage, ybc):
+def test_two_step_z_segments(gpu, rows, storage, ybc):
     """Tiles cut into z segments (each recomputes the first step of the plane below it):
     1 (the uncut kernel), 2, 3, 5 and 16 segments of a 17-plane sweep (down to one plane per
     segment, the last ending at the free top), 7 steps (the single-step launch cut the same
-    way).  Populations bit-identical to the oracle; force rows bit-identical to the uncut
-    sweep."""
+    way), with 12-row tiles (the next plane's loads in flight across each barrier: the
+    default for this lattice) and 13-row tiles (loads issued after the barrier).
+    Populations bit-identical to the oracle; force rows bit-identical to the uncut sweep."""
     from stonewall3d import _native as nat
     flag, uin = stones(200, 31, 18, 12, 60)
-    assert nat.lib().sw_get_step2_segments(200, 31, 18) > 1       # 13-row tiles, cut by default
+    assert nat.lib().sw_get_step2_segments(200, 31, 18) > 1       # fewer tiles than CUs: cut by default
     gust = gust_table(make_gust(8), 7)
     ref = None
-    for n in (1, 2, 3, 5, 16):
-        with step2_segments(n):
-            differ, Fg, Fo = compare(flag, uin, 7, storage, ybc, gust=gust)
-        assert differ == 0, f"{differ} populations differ with {n} segments"
-        if ref is None:
-            ref = Fg
-            assert np.allclose(Fg, Fo, rtol=1e-12, atol=1e-15 * max(np.abs(Fo).max(), 1e-300))
-        assert np.array_equal(Fg, ref), f"force rows with {n} segments differ from the uncut sweep"
+    nat.call("sw_set_step2_rows", rows)
+    try:
+        for n in (1, 2, 3, 5, 16):
+            with step2_segments(n):
+                differ, Fg, Fo = compare(flag, uin, 7, storage, ybc, gust=gust)
+            assert differ == 0, f"{differ} populations differ with {n} segments ({rows} rows)"
+            if ref is None:
+                ref = Fg
+                assert np.allclose(Fg, Fo, rtol=1e-12, atol=1e-15 * max(np.abs(Fo).max(), 1e-300))
+            assert np.array_equal(Fg, ref), f"force rows with {n} segments differ from the uncut sweep ({rows} rows)"
+    finally:
+        nat.call("sw_set_step2_rows", 0)
 
 
 @pytest.mark.parametrize("rows", [15, 16])
@@ -210,7 +216,7 @@ def test_two_step_z_segments(gpu, storage, ybc):
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_two_step_uncut_small_lattices(gpu, rows, dims, X0, storage):
     """The uncut kernel (15 or 16 rows, one workgroup per tile) on lattices that by default
-    take 13-row tiles cut into segments (fewer tiles than compute units): partial x and y
+    take 12-row tiles cut into segments (fewer tiles than compute units): partial x and y
     tiles, both side boundaries."""
     from stonewall3d import _native as nat
     nx, ny, nz = dims
@@ -241,4 +247,36 @@ def test_two_step_reference_lattice_segments(gpu):
     assert torch.equal(out[0][0].view(19, -1)[:, fl], out[1][0].view(19, -1)[:, fl])
     assert np.array_equal(out[0][1], out[1][1])
     del out
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_two_step_launch_shapes_agree_on_headline_lattice(gpu, storage):
+    """The 640x320x160 porous wall (250 tiles of 15 rows: uncut by default) stepped 4 times with
+    12-row tiles (next-plane loads in flight) cut into 4 segments, uncut 12-row tiles (two
+    rounds of workgroups) and 13-row tiles cut into 5 segments: fluid populations identical
+    to the default launch's, force rows equal to rounding (another per-tile summation)."""
+    from stonewall3d import _native as nat
+    w = workloads.porous_wall(H=40, porosity=0.2, seed=0, n_steps=4)
+    assert nat.lib().sw_get_step2_segments(640, 320, 160) == 1
+    fl = torch.from_numpy(w.flag == 0).cuda().permute(2, 1, 0).reshape(-1)
+    ref = None
+    try:
+        for rows, segs in ((0, 0), (12, 4), (12, 1), (13, 5)):
+            nat.call("sw_set_step2_rows", rows)
+            with step2_segments(segs):
+                sim = Simulation(w.flag, w.uin, dtype=storage, two_step=True, ring=4, **w.sim_kwargs())
+                sim.init(noise=0.02, seed=0)
+                sim.step(4)
+                f, F = sim.f[0].view(19, -1)[:, fl].clone(), sim.force_rows()
+                del sim
+            if ref is None:
+                ref = (f, F)
+                continue
+            assert torch.equal(ref[0], f), f"populations differ with {rows} rows, {segs} segments"
+            assert np.allclose(ref[1], F, rtol=1e-12, atol=1e-15 * np.abs(ref[1]).max())
+            del f
+    finally:
+        nat.call("sw_set_step2_rows", 0)
+    del ref
     torch.cuda.empty_cache()

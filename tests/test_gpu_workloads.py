@@ -129,7 +129,7 @@ def test_c3_batch_of_64_alone_equals_mix(gpu, dtype):
     old = nat.lib().sw_get_step_variant(code)
     old_rows = nat.lib().sw_get_step2_rows()
     nat.call("sw_set_step_variant", code, b.variant)
-    nat.call("sw_set_step2_rows", 16)
+    nat.call("sw_set_step2_rows", nat.lib().sw_batch_step2_rows())
     try:
         for lat, w, seed in lats:
             assert lat.t == steps
@@ -169,7 +169,7 @@ def test_c4_serving_trace_alone_equals_mix(gpu):
         old = nat.lib().sw_get_step_variant(code)
         old_rows = nat.lib().sw_get_step2_rows()
         nat.call("sw_set_step_variant", code, bt.variant)
-        nat.call("sw_set_step2_rows", 16)
+        nat.call("sw_set_step2_rows", nat.lib().sw_batch_step2_rows())
         try:
             sim = Simulation(w.flag, w.uin, dtype=req.dtype, two_step=bt.two_step, **w.sim_kwargs())
             sim.init(noise=0.02, seed=req.seed)

@@ -89,17 +89,21 @@ def test_two_step_sizes_and_argument_errors_without_device():
     """Host-side rules of the two-step entry points: partials sizes, the tile-rows setting,
     the batch table size, and argument errors raised before any device work."""
     L = nat.lib()
-    # the two-step batch's tiles: 64 x 14 cells; a lattice's ring holds either layout
-    assert L.sw_batch_partials_per_step(640, 320, 160) == 160 * (10 * 23) * 2
+    # the two-step batch's tiles: 64 x 10 cells (12 rows with their halo rows); a lattice's
+    # ring holds either layout
+    assert L.sw_batch_step2_rows() == 12
+    assert L.sw_batch_partials_per_step(640, 320, 160) == 160 * (10 * 32) * 2
     assert L.sw_batch_partials_per_step(128, 64, 32) == 32 * 16 * 2          # the row groups are more
     assert L.sw_batch_partials_per_step(0, 8, 8) == -1
     inst = ctypes.sizeof(nat.SwInstance)
     assert inst % 8 == 0
     assert L.sw_batch_table_bytes(4) == 64 + 4 * (inst + 2 * 136)           # + two argument records each
-    # tile rows: 0 (automatic), 15 or 16
+    # tile rows: 0 (automatic), 12, 13, 15 or 16
     assert L.sw_get_step2_rows() == 0
     assert L.sw_set_step2_rows(17) == 1001
     assert b"15 or 16" in L.sw_last_error()
+    assert L.sw_set_step2_rows(14) == 1001
+    assert L.sw_set_step2_rows(12) == 0 and L.sw_get_step2_rows() == 12
     assert L.sw_set_step2_rows(16) == 0 and L.sw_get_step2_rows() == 16
     assert L.sw_set_step2_rows(0) == 0 and L.sw_get_step2_rows() == 0
     fake = ctypes.c_void_p(0x1000)
