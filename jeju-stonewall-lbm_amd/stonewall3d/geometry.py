@@ -234,35 +234,45 @@ def rounded_box_wall(nx, ny, nz, X0, T, H, void_fraction, seed, size=(0.3, 0.6))
     yv = np.arange(ny) + 0.5
     kv = np.arange(nz) - 0.5
     xc, hx = X0 + T / 2, T / 2
-    masks = []
+    # Stone cells lie in the x window [xa, xb) and heights k >= 1.  Each stone's cells are kept
+    # as flat indices into a coverage count over that window (how many kept stones hold the
+    # cell); removing a stone decrements its cells, so the void fraction after each removal
+    # is updated from the stone's own cells instead of recomposing the whole mask.
+    xa, xb = max(0, X0 - 1), min(nx, X0 + T + 1)
+    cells, in_env = [], []
     for st in stones:
         r = 0.25 * min(hx, st["L"] / 2, st["h"] / 2)
-        xi = np.arange(max(0, X0 - 1), min(nx, X0 + T + 1))
+        xi = np.arange(xa, xb)
         ki = np.arange(max(1, int(st["zc"] - st["h"] / 2) - 1), min(nz, int(st["zc"] + st["h"] / 2) + 2))
         yi = np.arange(int(st["yc"] - st["L"] / 2) - 1, int(st["yc"] + st["L"] / 2) + 2)
         dy = (yv[yi % ny] - st["yc"] + ny / 2) % ny - ny / 2
         ins = _rounded_box(x[xi][:, None, None] - xc, dy[None, :, None], kv[ki][None, None, :] - st["zc"],
                            hx, st["L"] / 2, st["h"] / 2, r)
         ii, jj, kk = np.nonzero(ins)
-        masks.append((xi[ii], yi[jj] % ny, ki[kk]))
+        xs, ks = xi[ii], ki[kk]
+        flat = np.unique(((xs - xa) * ny + yi[jj] % ny) * nz + ks)
+        cells.append(flat)
+        xf, kf = flat // (ny * nz) + xa, flat % nz
+        in_env.append(flat[(xf >= X0) & (xf < X0 + T) & (kf >= 1) & (kf <= H)])
     env_cells = T * ny * H
+    cov = np.zeros((xb - xa) * ny * nz, np.int32)
+    for flat in cells:
+        cov[flat] += 1
+    covered = int(np.count_nonzero(cov.reshape(xb - xa, ny, nz)[X0 - xa:X0 - xa + T, :, 1:H + 1]))
 
-    def compose(keep):
-        m = np.zeros((nx, ny, nz), bool)
-        for k in keep:
-            xs, ys, ks = masks[k]
-            m[xs, ys, ks] = True
-        return m
-
-    def void(m):
-        return 1.0 - m[X0:X0 + T, :, 1:H + 1].sum() / env_cells
+    def void():
+        return 1.0 - covered / env_cells
 
     order = list(rng.permutation(len(stones)))
     keep = set(range(len(stones)))
-    m = compose(keep)
-    while void(m) < void_fraction and order:
-        keep.discard(order.pop(0))
-        m = compose(keep)
-    info = dict(stones=len(stones), removed=len(stones) - len(keep), void_fraction=float(void(m)),
-                frontal_porosity=float(frontal_porosity(m.any(axis=0), H)))
+    while void() < void_fraction and order:
+        k = order.pop(0)
+        keep.discard(k)
+        cov[cells[k]] -= 1
+        covered -= int(np.count_nonzero(cov[in_env[k]] == 0))
+    m = np.zeros((nx, ny, nz), bool)
+    win = cov.reshape(xb - xa, ny, nz) > 0
+    m[xa:xb] = win
+    info = dict(stones=len(stones), removed=len(stones) - len(keep), void_fraction=float(void()),
+                frontal_porosity=float(frontal_porosity(win.any(axis=0), H)))
     return m, info
