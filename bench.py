@@ -29,6 +29,8 @@ at the GPU's settled clocks; the CPU baseline runs last.
   cpu_baseline          the FP32 oracle (a C restatement of the reference kernel) on the
                         host: a bounded sample of the same lattice on all cores (value) and
                         on one thread, plus the 160 x 80 x 40 box-wall case
+  box_wall_160x80x40_gpu  that box-wall case's 2000 steps on the GPU (a lattice of 0.5 M
+                        cells: bound by launch latency and its few tiles, not by HBM)
 """
 import argparse
 import json
@@ -134,6 +136,32 @@ def timed_steps(w, dtype, shift, warmup, steps, ring=10, two_step=False):
     torch.cuda.synchronize()
     main = [a.elapsed_time(b) for (a, b), n in zip(ev, chunks) if n == per] or [a.elapsed_time(b) for a, b in ev]
     return sim, t_start.elapsed_time(t_end), float(np.mean(main)), per
+
+
+def box_wall_gpu(steps=2000, warmup=20):
+    """The CPU baseline's box-wall case (160 x 80 x 40, H = 10, FP32) for `steps` steps on the
+    GPU, as the CLI runs a lattice: two steps per launch, the force rows reduced every 64
+    steps; HIP events around the whole run.  Returns (MLUPS, ms per step)."""
+    import numpy as np
+    import torch
+    from stonewall3d import workloads
+    from stonewall3d.lbm3d import Simulation
+    w = workloads.box_wall_flow(n_steps=steps + warmup)
+    sim = Simulation(w.flag, w.uin, dtype="f32", ring=64, two_step=True, **w.sim_kwargs())
+    sim.init()
+    sim.step(warmup)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    sim.step(steps)
+    b.record(s)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    if not np.isfinite(sim.force_rows()).all():
+        raise SystemExit("box-wall force rows are not finite")
+    del sim
+    return w.cells / (ms * 1e-3) / 1e6, ms
 
 
 def copy_ceiling(nbytes, reps=10):
@@ -301,6 +329,13 @@ def main():
                                 "traffic": tr16, "traffic_source": src16, "storage_offset": kind16}
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
+        # the same box-wall case as the CPU baseline's second sample, on the GPU
+        gm, gms = box_wall_gpu()
+        cpu_bw = line["cpu_baseline"]["box_wall_160x80x40"]["all_cores"]["value"]
+        line["box_wall_160x80x40_gpu"] = {"value": round(gm, 1), "unit": "MLUPS", "steps": 2000,
+                                          "ms_per_step": round(gms, 5), "kernel": "k_step2",
+                                          "vs_cpu_all_cores": round(gm / cpu_bw, 1),
+                                          "what": "160x80x40 box wall H=10, FP32, 2000 steps, two per launch"}
     print(json.dumps(line), flush=True)
 
 if __name__ == "__main__":
