@@ -154,12 +154,13 @@ int sw_get_step2_rows(void);
  * rows and columns a tile reads are read by its neighbours in the same L2; 0 numbers them
  * plainly.  Results do not depend on it. */
 int sw_set_step2_tile_order(int banded);
-/* z segments per tile of the two-step kernels with 12- or 13-row tiles (no reference counterpart):
- * each tile's sweep is cut into this many workgroups (1 .. 32, at most nz - 1), each
- * recomputing the first step of the plane below its segment, so that a lattice with fewer
- * tiles than compute units fills the device.  0 (default) picks the count per lattice.
- * Lattices with 15- or 16-row tiles are not cut.  Populations and force rows do not depend on
- * it.  sw_get_step2_segments returns the count a lattice gets (-1 on bad dimensions). */
+/* z segments per tile of the two-step kernels with 12- or 13-row tiles (no reference
+ * counterpart): each tile's sweep is cut into this many workgroups (1 .. 32, at most
+ * nz - 1), each recomputing the first step of the plane below its segment, so that a
+ * lattice with fewer tiles than compute units fills the device.  0 (default) picks the
+ * count per lattice.  Lattices with 15- or 16-row tiles are not cut.  Populations and force
+ * rows do not depend on it.  sw_get_step2_segments returns the count a lattice gets (-1 on
+ * bad dimensions). */
 int sw_set_step2_segments(int segments);
 int sw_get_step2_segments(int nx, int ny, int nz);
 
