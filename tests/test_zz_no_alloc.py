@@ -4,9 +4,6 @@ array) against sw_step + sw_force_reduce, through the C ABI (needs an MI355X).
 Two lattices in a row, the second smaller, several steps each: populations and force rows
 identical to the allocating entry points; a lattice whose partials exceed the fixed array
 is refused before anything is launched."""
-import ctypes
-
-import numpy as np
 import pytest
 import torch
 
