@@ -237,8 +237,8 @@ def main():
     nx, ny, nz = w.shape
     N = nx * ny * nz
     # the fastest kernel of each storage form on this lattice: two steps per launch for FP32
-    # (0.79 against 0.65 of 8 TB/s), one step per launch for the 16-bit forms (the two-step
-    # form of those is limited by registers and VALU and runs at about half their one-step rate)
+    # (0.90 against 0.64 of 8 TB/s), one step per launch for the 16-bit forms (the two-step
+    # form of those runs at about two thirds of their one-step rate)
     two = (a.dtype == "f32" and not a.one_step and two_step_ok(w, a.dtype))
     bpu = bytes_per_update(a.dtype)
     headline = a.workload == "porous_wall" and a.H == 40 and a.porosity == 0.2

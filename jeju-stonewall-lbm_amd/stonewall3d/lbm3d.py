@@ -486,8 +486,8 @@ def main(argv=None):
           f"{3*su.nu0+0.5:.5f}), Re {a.Re:.0f}; {n_total} steps, one H/U = {su.t_unit:.0f} steps", flush=True)
     wall, phi = build_wall(a.porosity, H, nx, ny, nz, X0, T, seed=a.seed)
     gust = gust_table(make_gust(a.gust_seed), n_total) if a.gust_seed is not None else None
-    # two steps per launch where it is the faster kernel: FP32 below 4 GiB per array (0.78 against
-    # 0.64 of 8 TB/s on 640x320x160); the 16-bit two-step form is VALU-bound at about half the
+    # two steps per launch where it is the faster kernel: FP32 below 4 GiB per array (0.90 against
+    # 0.64 of 8 TB/s on 640x320x160); the 16-bit two-step form runs at about two thirds of the
     # rate of their one-step kernel, and FP64 has no two-step form
     two = a.dtype == "f32" and not a.one_step and Q * nx * ny * nz * nat.DTYPE_BYTES[nat.DTYPES[a.dtype]] < (1 << 32)
     sim = Simulation(wall, su.uin, nu0=su.nu0, cs=a.Cs, x_sponge=su.x_sponge, nu_sponge=0.05, ybc=a.ybc,
