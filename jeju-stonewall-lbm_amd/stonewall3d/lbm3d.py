@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import npz
 from .lattice import FLUID, GROUND, Q, WALL, flag_from_mask
 
 TORCH_STORAGE = {nat.SW_F32: torch.float32, nat.SW_F16: torch.float16, nat.SW_BF16: torch.bfloat16,
@@ -570,7 +571,7 @@ def main(argv=None):
     mean = sim.mean_field()
     if mean is None:
         mean = np.zeros((4, nx, ny, nz), np.float32)
-    np.savez_compressed(
+    npz.savez_compressed(                     # np.savez_compressed's archive, deflated on threads
         os.path.join(a.out, f"{a.case}.npz"),
         Cd=(Fx / (qref * A)).astype(np.float32), Cl=(Fz / (qref * A)).astype(np.float32),
         Cm=(M / (qref * A * H)).astype(np.float32),
