@@ -136,6 +136,15 @@ int sw_step2(const void* f_in, void* f_out, int dtype, const uint32_t* links, co
 int sw_step2_one(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin, const float* gust,
                  int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs, int x_sponge,
                  double nu_sponge, int ybc, float scale, const float* shift, double* partials, int ring, void* stream);
+/* sw_step2 that also adds (rho, ux, uy, uz) of every cell of the state after the pair's first
+ * step (which = 1: step t + 1) or second step (which = 2: step t + 2) into acc[4][N], exactly
+ * as sw_macro(..., Uacc = acc) on that state would (solid cells add (1, 0, 0, 0)), without a
+ * pass over the populations: the averaging launches k_macro + k_accum of lbm3d.py:363-370 on
+ * a sampled step inside a pair.  FP32 storage only (the form run two steps per launch). */
+int sw_step2_accumulate(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
+                        const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
+                        int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
+                        int ring, float* acc, int which, void* stream);
 /* force[t][z][0..1] from the sw_step2 partials ring, as sw_force_reduce does for sw_step's. */
 int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int nx, int ny, int nz,
                      double* force, void* stream);

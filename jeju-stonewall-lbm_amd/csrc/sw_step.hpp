@@ -214,7 +214,7 @@ __device__ __forceinline__ void collide(const T (&fl)[Q], uint32_t x, A& a, T sm
 // sequential sums in direction order (FP32 / FP64) or the opposite-pair moments of the
 // 16-bit forms, u = m / rho by IEEE division, acc[k] = acc[k] + u[k].
 template <bool NORM, class T>
-__device__ __forceinline__ void macro_accumulate(float* acc, size_t N, size_t cell, const T (&sv)[Q]) {
+__device__ __forceinline__ void macro_values(const T (&sv)[Q], float (&u)[4]) {
     T rho = T(0), mx = T(0), my = T(0), mz = T(0);
     if constexpr (NORM) {
         const Moments<T> m = moments_norm<T, false>(sv);
@@ -229,7 +229,12 @@ __device__ __forceinline__ void macro_accumulate(float* acc, size_t N, size_t ce
             if constexpr (CZ[i] != 0) mz = CZ[i] > 0 ? mz + v : mz - v;
         });
     }
-    const float u[4] = {(float)rho, (float)(mx / rho), (float)(my / rho), (float)(mz / rho)};
+    u[0] = (float)rho; u[1] = (float)(mx / rho); u[2] = (float)(my / rho); u[3] = (float)(mz / rho);
+}
+template <bool NORM, class T>
+__device__ __forceinline__ void macro_accumulate(float* acc, size_t N, size_t cell, const T (&sv)[Q]) {
+    float u[4];
+    macro_values<NORM, T>(sv, u);
 #pragma unroll
     for (int k = 0; k < 4; k++) acc[k * N + cell] = acc[k * N + cell] + u[k];
 }

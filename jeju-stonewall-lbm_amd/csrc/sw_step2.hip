@@ -11,11 +11,11 @@ __device__ __forceinline__ int step2_tile_of(int b, int ntiles, bool banded) {
     return band_start(b % NXCD, ntiles) + b / NXCD;
 }
 
-template <class S, int TY, bool PF = false>
+template <class S, int TY, bool PF = false, int ACC = 0>
 __global__ __launch_bounds__(64 * TY, 1) void k_step2(StepArgs a, int ntx, int ntiles, int banded) {
     __shared__ Step2Lds<TY> L;
-    step2_tile<S, TY, false, StepArgs, PF>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles,
-                                           step2_tile_of((int)blockIdx.x, ntiles, banded != 0), L);
+    step2_tile<S, TY, false, StepArgs, PF, ACC>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles,
+                                                step2_tile_of((int)blockIdx.x, ntiles, banded != 0), L);
 }
 
 template <class S, int TY>
@@ -32,15 +32,15 @@ __global__ __launch_bounds__(64 * TY, 1) void k_step2_one(StepArgs a, int ntx, i
 // tiles: with a run-time za (step2_tile) the kernel needs 128 VGPRs, which 13 wavefronts per
 // workgroup allow without spilling (at 15 or 16 rows it spilled about 30 and ran 1.46x slower
 // per plane).
-template <class S, int TY, bool SINGLE, bool PF = false>
+template <class S, int TY, bool SINGLE, bool PF = false, int ACC = 0>
 __global__ __launch_bounds__(64 * TY, 1) void k_step2_seg(StepArgs a, int ntx, int ntiles, int nseg, int banded) {
     __shared__ Step2Lds<TY> L;
     const int item = step2_tile_of((int)blockIdx.x, ntiles * nseg, banded != 0);
     const int g = item / ntiles;
     const int za = 1 + (int)(((long long)g * (a.nz - 1)) / nseg);
     const int zb = 1 + (int)(((long long)(g + 1) * (a.nz - 1)) / nseg);
-    step2_tile<S, TY, SINGLE, StepArgs, PF>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, item - g * ntiles, L,
-                                            za, zb);
+    step2_tile<S, TY, SINGLE, StepArgs, PF, ACC>(a, a.t + (a.t_dev ? *a.t_dev : 0), ntx, ntiles, item - g * ntiles,
+                                                 L, za, zb);
 }
 
 constexpr int SEG_ROWS = 13;          // tile rows of the segmented launch
@@ -98,33 +98,34 @@ static int step2_segments(int nx, int ny, int nz, int cus, int ty) {
     return step2_auto_segments(nx, ny, nz, cus, ty);
 }
 
-template <class S, bool SINGLE>
+template <class S, bool SINGLE, int ACC = 0>
 static void launch_step2(const StepArgs& a, int cus, hipStream_t s) {
+    static_assert(!(SINGLE && ACC), "the accumulating form is a two-step sweep");
     const int ntx = (a.nx + 63) / 64;
     const int ty = step2_rows(a.nx, a.ny, a.nz, cus);
     const int tiles = step2_tiles_for(a.nx, a.ny, ty);
     if (ty == PF_ROWS) {
         const int nseg = step2_segments(a.nx, a.ny, a.nz, cus, ty);
         if (nseg > 1) {
-            hipLaunchKernelGGL((k_step2_seg<S, PF_ROWS, SINGLE, !SINGLE>), dim3((unsigned)(tiles * nseg)),
+            hipLaunchKernelGGL((k_step2_seg<S, PF_ROWS, SINGLE, !SINGLE, ACC>), dim3((unsigned)(tiles * nseg)),
                                dim3(64 * PF_ROWS), 0, s, a, ntx, tiles, nseg, g_step2_banded);
         } else if constexpr (SINGLE) {
             hipLaunchKernelGGL((k_step2_one<S, PF_ROWS>), dim3(tiles), dim3(64 * PF_ROWS), 0, s, a, ntx, tiles,
                                g_step2_banded);
         } else {
-            hipLaunchKernelGGL((k_step2<S, PF_ROWS, true>), dim3(tiles), dim3(64 * PF_ROWS), 0, s, a, ntx, tiles,
-                               g_step2_banded);
+            hipLaunchKernelGGL((k_step2<S, PF_ROWS, true, ACC>), dim3(tiles), dim3(64 * PF_ROWS), 0, s, a, ntx,
+                               tiles, g_step2_banded);
         }
     } else if (ty == SEG_ROWS) {
         const int nseg = step2_segments(a.nx, a.ny, a.nz, cus, ty);
-        hipLaunchKernelGGL((k_step2_seg<S, SEG_ROWS, SINGLE>), dim3((unsigned)(tiles * nseg)), dim3(64 * SEG_ROWS), 0,
-                           s, a, ntx, tiles, nseg, g_step2_banded);
+        hipLaunchKernelGGL((k_step2_seg<S, SEG_ROWS, SINGLE, false, ACC>), dim3((unsigned)(tiles * nseg)),
+                           dim3(64 * SEG_ROWS), 0, s, a, ntx, tiles, nseg, g_step2_banded);
     } else if (ty == 15) {
         if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles, g_step2_banded);
-        else hipLaunchKernelGGL((k_step2<S, 15>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles, g_step2_banded);
+        else hipLaunchKernelGGL((k_step2<S, 15, false, ACC>), dim3(tiles), dim3(64 * 15), 0, s, a, ntx, tiles, g_step2_banded);
     } else {
         if constexpr (SINGLE) hipLaunchKernelGGL((k_step2_one<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles, g_step2_banded);
-        else hipLaunchKernelGGL((k_step2<S, 16>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles, g_step2_banded);
+        else hipLaunchKernelGGL((k_step2<S, 16, false, ACC>), dim3(tiles), dim3(64 * 16), 0, s, a, ntx, tiles, g_step2_banded);
     }
 }
 
@@ -144,7 +145,7 @@ __global__ void k_force_reduce2(const double* __restrict__ partials, int ring, i
 static int step2_entry(bool single, const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
                        const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
                        int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
-                       int ring, void* stream) {
+                       int ring, void* stream, float* acc = nullptr, int which = 0) {
     SW_CHECK_ARG(f_in && f_out && links && uin, "f_in, f_out, links and uin must be non-null");
     SW_CHECK_ARG(f_in != f_out, "f_in and f_out must be different buffers (pull scheme)");
     SW_CHECK_RC(check_dims(nx, ny, nz));
@@ -155,6 +156,8 @@ static int step2_entry(bool single, const void* f_in, void* f_out, int dtype, co
     SW_CHECK_ARG(ybc == Y_PERIODIC || ybc == Y_FREESLIP, "ybc must be 0 (periodic) or 1 (free-slip)");
     SW_CHECK_ARG(!partials || ring >= 2, "ring must be at least 2 when partials are given");
     SW_CHECK_ARG(t >= 0, "t must be non-negative");
+    SW_CHECK_ARG(which == 0 || (!single && acc && dtype == DT_F32 && (which == 1 || which == 2)),
+                 "the accumulating two-step launch takes FP32 storage, a non-null acc and which = 1 or 2");
     int cus = 0;
     SW_CHECK_RC(device_cus(&cus));
     StepArgs a{};
@@ -164,7 +167,18 @@ static int step2_entry(bool single, const void* f_in, void* f_out, int dtype, co
     a.nx = nx; a.ny = ny; a.nz = nz; a.nyb = (ny + ROWS - 1) / ROWS;
     a.x_sponge = x_sponge; a.ybc = ybc; a.nu0 = nu0; a.cs = cs; a.nu_sponge = nu_sponge;
     a.scale = scale; a.inv_scale = 1.0f / scale;
+    a.acc = acc;
     hipStream_t s = (hipStream_t)stream;
+    if (which == 1) {
+        launch_step2<float, false, 1>(a, cus, s);
+        SW_HIP(hipGetLastError());
+        return SW_OK;
+    }
+    if (which == 2) {
+        launch_step2<float, false, 2>(a, cus, s);
+        SW_HIP(hipGetLastError());
+        return SW_OK;
+    }
     switch (dtype) {
         case DT_F32: single ? launch_step2<float, true>(a, cus, s) : launch_step2<float, false>(a, cus, s); break;
         case DT_F16: single ? launch_step2<_Float16, true>(a, cus, s) : launch_step2<_Float16, false>(a, cus, s); break;
@@ -227,6 +241,14 @@ int sw_step2_one(const void* f_in, void* f_out, int dtype, const uint32_t* links
                  double nu_sponge, int ybc, float scale, const float* shift, double* partials, int ring, void* stream) {
     return step2_entry(true, f_in, f_out, dtype, links, uin, gust, t, t_dev, nx, ny, nz, nu0, cs, x_sponge,
                        nu_sponge, ybc, scale, shift, partials, ring, stream);
+}
+
+int sw_step2_accumulate(const void* f_in, void* f_out, int dtype, const uint32_t* links, const float* uin,
+                        const float* gust, int t, const int* t_dev, int nx, int ny, int nz, double nu0, double cs,
+                        int x_sponge, double nu_sponge, int ybc, float scale, const float* shift, double* partials,
+                        int ring, float* acc, int which, void* stream) {
+    return step2_entry(false, f_in, f_out, dtype, links, uin, gust, t, t_dev, nx, ny, nz, nu0, cs, x_sponge,
+                       nu_sponge, ybc, scale, shift, partials, ring, stream, acc, which);
 }
 
 int sw_force_reduce2(const double* partials, int ring, int t0, const int* t_dev, int nsteps, int nx, int ny, int nz,

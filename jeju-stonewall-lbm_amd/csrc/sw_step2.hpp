@@ -70,7 +70,11 @@ __device__ __forceinline__ float& lds_row(Step2Lds<TY>& L, int j, int plane, int
 // k, before the barrier, so they are in flight during the barrier and the second step (19
 // more registers held across phase 2: with fewer than 13 rows the workgroup runs at most 3
 // wavefronts per SIMD, which leaves 168 VGPRs for them).
-template <class S, int TY, bool SINGLE, class A, bool PF = false>
+// ACC (two-step sweep only): 1 or 2 adds (rho, u) of every cell's state after the pair's
+// first or second step into a.acc, as sw_macro's accumulate would on that state
+// (lbm3d.py:158-188): the output rows sample the planes [za, zb) they own, the segment with
+// za = 1 also the solid ground plane.
+template <class S, int TY, bool SINGLE, class A, bool PF = false, int ACC = 0>
 __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int tile, Step2Lds<TY>& L, int za = 1,
                                            int zb = 1 << 30) {
     using Cd = Codec<S>;
@@ -179,6 +183,22 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         } else {
             return v;
         }
+    };
+    // (ACC) one cell's sample: the accumulator's four values are loaded at the start of the
+    // phase that computes the state (acc_fetch), so that their latency passes during its
+    // collision, and prev + (rho, u) of the stored populations sv, or + (1, 0, 0, 0) for a
+    // solid cell (sw_macro reads a solid cell so), is stored at its end (acc_sample)
+    auto acc_fetch = [&](int k, float (&prev)[4]) {
+        const size_t cell = ((size_t)k * ny + y) * nx + x;
+#pragma unroll
+        for (int c = 0; c < 4; c++) prev[c] = a.acc[c * (size_t)N + cell];
+    };
+    auto acc_sample = [&](int k, const float (&prev)[4], const T (&sv)[Q], bool live) {
+        const size_t cell = ((size_t)k * ny + y) * nx + x;
+        float u[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+        if (live) macro_values<NORM, T>(sv, u);
+#pragma unroll
+        for (int c = 0; c < 4; c++) a.acc[c * (size_t)N + cell] = prev[c] + u[c];
     };
     auto chunk_fast = [&](int row, int k, int c) -> bool {
         return !((ld_const(cflags, ((uint32_t)k * ny + row) * cw + (c >> 5)) >> (c & 31)) & 1u);
@@ -342,6 +362,7 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         static_for<0, 3>([&](auto C) { en[C] = (T)neg_scaled((float)sig(k, C), kscale); });
     };
 
+    static_assert(!(SINGLE && ACC), "the accumulating form is a two-step sweep");
     if constexpr (SINGLE) {
         // ---- one step: the output rows compute f1 of every plane and store it; the force of
         // step t0 goes to slot t0 % ring in the per-tile layout, summed as the pair's first step
@@ -417,6 +438,9 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
             p[0] = 0.0; p[1] = 0.0;
         }
     }
+    if constexpr (ACC != 0) {
+        if (out_row && za == 1 && x < nx) solid_accumulate(a.acc, (size_t)N, (size_t)y * nx + x);   // ground plane
+    }
     if (halo_wave && ks < nz) halo_f1(ks);
     B pre[Q];                                                // PF: streamed loads of the next plane
     if constexpr (PF) {
@@ -433,6 +457,11 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         T pk[Q];                                  // what the second step reads of f1(k)
         uint32_t m_cur = 0;
         if (need && k < nz) {
+            const bool take1 = ACC == 1 && out_row && x < nx && k >= za && k < zb;
+            float prev1[4];
+            if constexpr (ACC == 1) {
+                if (take1) acc_fetch(k, prev1);
+            }
             T fl[Q];
             T fxw = T(0), fzw = T(0);
             bool live = true;
@@ -452,6 +481,9 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
                 static_for<0, Q>([&](auto I) { v[I] = inlet_value<NORM, I, T>(u0, usq15); });
             }
             static_for<0, Q>([&](auto I) { pk[I] = stored(I, v[I], k); });
+            if constexpr (ACC == 1) {
+                if (take1) acc_sample(k, prev1, pk, live);
+            }
             if (force) wave_force(out_row && x < nx && x != 0 && live ? (float)fxw : 0.0f,
                                   out_row && x < nx && x != 0 && live ? (float)fzw : 0.0f, k % 3, 0);
             static_for<0, Q>([&](auto I) {
@@ -473,6 +505,10 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
         float fbx = 0.0f, fbz = 0.0f;
         if (out_row && kk >= za) {
             const bool top = kk == nz - 1;
+            float prev2[4];
+            if constexpr (ACC == 2) {
+                if (x < nx) acc_fetch(kk, prev2);
+            }
             T g[Q];
             auto shr = [&](T v, T edge) { return __uint_as_float(wave_shr1(__float_as_uint(v), __float_as_uint(edge))); };
             auto shl = [&](T v, T edge) {
@@ -544,20 +580,26 @@ __device__ __forceinline__ void step2_tile(A& a, int t0, int ntx, int P, int til
             };
             const bool st_ok = x < nx && live;
             const uint32_t rowb = ((uint32_t)kk * ny + (uint32_t)y) * (uint32_t)nx * ES;
+            T sv[ACC == 2 ? Q : 1];                                          // (ACC 2) the stored state
             if (x == 0) {                                                    // inlet
                 const T u0 = inlet_u0(kk, t0 + 1);
                 const T usq15 = T(1.5) * (u0 * u0);
                 static_for<0, Q>([&](auto I) {
                     constexpr int i = I;
-                    if (st_ok) buf_store<B, 2>(Cd::encode(inlet_value<NORM, i, T>(u0, usq15), esh(I), scale), rout, xo,
-                                               (uint32_t)i * NB + rowb);
+                    const T val = inlet_value<NORM, i, T>(u0, usq15);
+                    if (st_ok) buf_store<B, 2>(Cd::encode(val, esh(I), scale), rout, xo, (uint32_t)i * NB + rowb);
+                    if constexpr (ACC == 2) sv[i] = stored(I, val, kk);
                 });
             } else {
                 collide<NORM, T>(g, (uint32_t)xl, a, smag, [&](auto I, T val) {
                     constexpr int i = I;
                     if (st_ok) buf_store<B, 2>(Cd::encode(val, esh(I), scale), rout, xo, (uint32_t)i * NB + rowb);
+                    if constexpr (ACC == 2) sv[i] = stored(I, val, kk);
                 });
                 if (st_ok) { fbx = (float)fxw; fbz = (float)fzw; }
+            }
+            if constexpr (ACC == 2) {
+                if (x < nx) acc_sample(kk, prev2, sv, live);
             }
         }
         if (halo_wave && k + 1 < nz && k < ke) halo_f1(k + 1);

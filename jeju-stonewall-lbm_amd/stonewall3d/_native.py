@@ -37,6 +37,9 @@ SIGNATURES = {
                           _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p]),
     "sw_step2_one": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
                               _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p]),
+    "sw_step2_accumulate": (_c_int, [_p, _p, _c_int, _p, _p, _p, _c_int, _p, _c_int, _c_int, _c_int,
+                                     _c_dbl, _c_dbl, _c_int, _c_dbl, _c_int, _c_flt, _p, _p, _c_int, _p, _c_int,
+                                     _p]),
     "sw_force_reduce2": (_c_int, [_p, _c_int, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "sw_set_step2_rows": (_c_int, [_c_int]),
     "sw_get_step2_rows": (_c_int, []),

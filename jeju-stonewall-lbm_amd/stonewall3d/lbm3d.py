@@ -172,57 +172,57 @@ class Simulation:
         self._pending = 0
         return self
 
-    def step(self, n=1, accumulate=False):
+    def step(self, n=1, accumulate=False, samples=None):
         """n fused steps (lbm3d.py:358-362); with two_step, pairs of them per launch.
-        accumulate: the last of them also adds (rho, u) of its result into the time-mean
-        accumulator (k_macro + k_accum after k_step, lbm3d.py:363-370) in the same launch."""
+        samples: offsets among the n steps (0 = the first) whose resulting state is added to
+        the time-mean accumulator (k_macro + k_accum after k_step, lbm3d.py:363-370) by the
+        launch that computes it; accumulate=True samples the last of them.  With two_step and
+        FP32 storage a sampled step inside a pair stays in the pair (sw_step2_accumulate);
+        otherwise it runs alone (sw_step_accumulate)."""
         lib = nat.lib()
         s = _stream()
         n = int(n)
         if self.t + n > self.n_steps:
             raise RuntimeError(f"step {self.t + n - 1} beyond n_steps={self.n_steps}")
-        if accumulate and n > 0 and self.Uacc is None:
+        samples = {int(k) for k in (samples or ())} | ({n - 1} if accumulate and n > 0 else set())
+        if any(k < 0 or k >= n for k in samples):
+            raise ValueError("samples must lie in [0, n)")
+        if samples and self.Uacc is None:
             self.Uacc = torch.zeros(4 * self.N, dtype=torch.float32, device=self.device)
-        while n > 0:
-            pair = self.two_step and (n >= 3 or (n == 2 and not accumulate))
-            # with two_step, a single step other than the accumulating one takes the two-step
-            # kernel's tiles (sw_step2_one), so its partials share the pairs' ring layout
-            single2 = self.two_step and not pair and not (accumulate and n == 1)
+        acc_pairs = self.two_step and self.dtype == nat.SW_F32
+        i = 0
+        while i < n:
+            s0, s1 = i in samples, (i + 1) in samples
+            pair = self.two_step and n - i >= 2 and not (s0 and s1) and (acc_pairs or not (s0 or s1))
+            # with two_step, a single step other than a sampled one takes the two-step kernel's
+            # tiles (sw_step2_one), so its partials share the pairs' ring layout
+            single2 = self.two_step and not pair and not s0
             layout = 2 if (pair or single2) else 1
             adv = 2 if pair else 1
             if self.record_force and self._pending and (self._layout != layout or
                                                         self._pending + adv > self.ring):
                 self.flush_force()
             self._layout = layout
+            common = (self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
+                      self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz, self.nu0,
+                      self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale, _ptr(self.shift))
             if single2:
-                rc = lib.sw_step2_one(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
-                                      self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
-                                      self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
-                                      _ptr(self.shift), _ptr(self.partials2), self.ring, s)
-                nat.check(rc, "sw_step2_one")
+                nat.check(lib.sw_step2_one(*common, _ptr(self.partials2), self.ring, s), "sw_step2_one")
+            elif pair and (s0 or s1):
+                nat.check(lib.sw_step2_accumulate(*common, _ptr(self.partials2), self.ring, self.Uacc.data_ptr(),
+                                                  1 if s0 else 2, s), "sw_step2_accumulate")
+                self.n_acc += 1
             elif pair:
-                rc = lib.sw_step2(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
-                                  self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
-                                  self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
-                                  _ptr(self.shift), _ptr(self.partials2), self.ring, s)
-                nat.check(rc, "sw_step2")
-            elif accumulate and n == 1:
-                rc = lib.sw_step_accumulate(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype,
-                                            self.links.data_ptr(), self.uin.data_ptr(), _ptr(self.gust), self.t, None,
-                                            self.nx, self.ny, self.nz, self.nu0, self.cs, self.x_sponge,
-                                            self.nu_sponge, self.ybc, self.scale, _ptr(self.shift),
-                                            _ptr(self.partials), self.ring, self.Uacc.data_ptr(), s)
-                nat.check(rc, "sw_step_accumulate")
+                nat.check(lib.sw_step2(*common, _ptr(self.partials2), self.ring, s), "sw_step2")
+            elif s0:
+                nat.check(lib.sw_step_accumulate(*common, _ptr(self.partials), self.ring, self.Uacc.data_ptr(), s),
+                          "sw_step_accumulate")
                 self.n_acc += 1
             else:
-                rc = lib.sw_step(self.f[0].data_ptr(), self.f[1].data_ptr(), self.dtype, self.links.data_ptr(),
-                                 self.uin.data_ptr(), _ptr(self.gust), self.t, None, self.nx, self.ny, self.nz,
-                                 self.nu0, self.cs, self.x_sponge, self.nu_sponge, self.ybc, self.scale,
-                                 _ptr(self.shift), _ptr(self.partials), self.ring, s)
-                nat.check(rc, "sw_step")
+                nat.check(lib.sw_step(*common, _ptr(self.partials), self.ring, s), "sw_step")
             self.f[0], self.f[1] = self.f[1], self.f[0]
             self.t += adv
-            n -= adv
+            i += adv
             if self.record_force:
                 self._pending += adv
                 if self._pending == self.ring:
@@ -522,21 +522,20 @@ def main(argv=None):
         return k + (-(k - base)) % period
 
     def next_event(t):
-        """The first step >= t after which the loop samples, snapshots, checkpoints or reports."""
-        ev = [n_total - 1, first_at_or_after(t, 0, 1000), first_at_or_after(t, n_spin, 5),
-              first_at_or_after(t, n_spin, snap_every)]
+        """The first step >= t after which the loop snapshots, checkpoints or reports."""
+        ev = [n_total - 1, first_at_or_after(t, 0, 1000), first_at_or_after(t, n_spin, snap_every)]
         if a.ckpt_every > 0:
             ev.append(first_at_or_after(t + 1, 0, a.ckpt_every) - 1)
         return min(e for e in ev if e >= t)
 
     # The steps between two events run as one call (pairs of steps per launch where the storage
-    # allows); a sampled step adds its state to the time mean in the same launch (lbm3d.py:363-370).
+    # allows); each sampled step (every 5th of the averaging window) adds its state to the time
+    # mean in the launch that computes it (lbm3d.py:363-370).
     t0 = time.time()
     t = t_start
     while t < n_total:
         e = next_event(t)
-        sampled = e >= n_spin and (e - n_spin) % 5 == 0
-        sim.step(e - t + 1, accumulate=sampled)
+        sim.step(e - t + 1, samples=[k - t for k in range(first_at_or_after(t, n_spin, 5), e + 1, 5)])
         t = e
         if t >= n_spin:
             k = t - n_spin

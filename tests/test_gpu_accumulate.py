@@ -72,3 +72,49 @@ def test_fused_accumulate_after_two_step_launches(gpu, storage):
     fl = torch.from_numpy(flag == 0).cuda().permute(2, 1, 0).reshape(-1)
     assert torch.equal(a.f[0].view(19, -1)[:, fl], b.f[0].view(19, -1)[:, fl])
     assert np.allclose(a.force_rows(), b.force_rows(), rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("rows,segs", [(0, 0), (15, 0), (16, 0), (13, 3), (12, 1), (12, 4)])
+def test_two_step_sampled_pairs_equal_macro_pass(gpu, rows, segs):
+    """FP32, two steps per launch, with samples at the first and at the second step of pairs
+    (sw_step2_accumulate), two adjacent samples (one of them in a launch of its own) and the
+    last step: the accumulator bit-identical to sw_macro after every sampled single step,
+    the populations identical and the force rows equal to rounding; every launch shape (the
+    automatic one, 15 and 16 rows uncut, 13 rows cut into 3 segments, 12 rows uncut and cut)."""
+    flag, uin = lattice(200, 90, inlet_stone=True, ny=31, nz=18)
+    nat.call("sw_set_step2_rows", rows)
+    nat.call("sw_set_step2_segments", segs)
+    try:
+        a, b = pair(flag, uin, "f32", 24, two_step=True)
+        sampled = {0, 3, 4, 9, 12, 17, 20, 21, 23}
+        for t in range(24):
+            a.step(1)
+            if t in sampled:
+                a.accumulate()
+        b.step(10, samples=[k for k in sampled if k < 10])
+        b.step(14, samples=[k - 10 for k in sampled if k >= 10])
+        assert a.n_acc == b.n_acc == len(sampled)
+        assert torch.equal(a.Uacc, b.Uacc)
+        fl = torch.from_numpy(flag == 0).cuda().permute(2, 1, 0).reshape(-1)
+        assert torch.equal(a.f[0].view(19, -1)[:, fl], b.f[0].view(19, -1)[:, fl])
+        assert np.allclose(a.force_rows(), b.force_rows(), rtol=1e-12, atol=1e-15)
+    finally:
+        nat.call("sw_set_step2_rows", 0)
+        nat.call("sw_set_step2_segments", 0)
+
+
+def test_two_step_accumulate_argument_errors(gpu):
+    """The accumulating pair takes FP32 only, a non-null accumulator and which = 1 or 2."""
+    flag, uin = lattice(48, 12)
+    _, b = pair(flag, uin, "f16", 4, two_step=True)
+    with pytest.raises(ValueError):
+        b.step(2, samples=[2])
+    acc = torch.zeros(4 * b.N, dtype=torch.float32, device="cuda")
+    args = (b.f[0].data_ptr(), b.f[1].data_ptr(), b.dtype, b.links.data_ptr(), b.uin.data_ptr(), None, 0, None,
+            b.nx, b.ny, b.nz, b.nu0, b.cs, b.x_sponge, b.nu_sponge, b.ybc, b.scale, b.shift.data_ptr(),
+            b.partials2.data_ptr(), b.ring)
+    assert nat.lib().sw_step2_accumulate(*args, acc.data_ptr(), 1, None) == 1001          # FP16
+    assert b"FP32" in nat.lib().sw_last_error()
+    # FP16 two-step runs keep sampled steps alone and stay correct
+    b.step(4, samples=[1, 2])
+    assert b.n_acc == 2
