@@ -113,6 +113,11 @@ def test_two_step_sizes_and_argument_errors_without_device():
     f64 = (3,) + args[1:]
     assert L.sw_step2_one(fake, ctypes.c_void_p(0x2000), *f64) == 1001
     assert b"FP32, FP16 or BF16" in L.sw_last_error()
+    acc_args = args[:-1]                                                    # up to ring; then acc, which, stream
+    assert L.sw_step2_accumulate(fake, ctypes.c_void_p(0x2000), *acc_args, None, 1, None) == 1001   # acc missing
+    assert L.sw_step2_accumulate(fake, ctypes.c_void_p(0x2000), *acc_args, fake, 3, None) == 1001   # which
+    assert L.sw_step2_accumulate(fake, ctypes.c_void_p(0x2000), *((1,) + acc_args[1:]), fake, 1, None) == 1001
+    assert b"FP32" in L.sw_last_error()
     assert L.sw_batch_step2(None, 0, 0, 256, None) == 1001
     assert L.sw_batch_step2(fake, 3, 0, 256, None) == 1001                   # FP64
     assert L.sw_batch_step2_tail(fake, 0, 7, 256, None) == 1001              # K odd
