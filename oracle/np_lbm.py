@@ -121,7 +121,8 @@ def step(f, flag, uin, nu0, cs=0.16, x_sponge=None, nu_sponge=0.05, gust=1.0, yb
 
 def macro(f, flag):
     rho = f.sum(0)
-    u = np.tensordot(C.T, f, axes=(1, 0)) / rho
+    with np.errstate(invalid="ignore", divide="ignore"):      # solid cells: rho = 0, set below
+        u = np.tensordot(C.T, f, axes=(1, 0)) / rho
     U = np.concatenate([rho[None], u])
     solid = flag != 0
     U[0][solid] = 1.0

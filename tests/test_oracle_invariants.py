@@ -288,7 +288,8 @@ def test_f16_codec_exhaustive(oracle_lib):
     x = np.concatenate([rng.standard_normal(20000).astype(np.float32) * s for s in (1e-7, 1e-5, 1e-3, 1, 1e3, 7e4)])
     x = np.concatenate([x, vals[fin], (vals[fin] + np.float32(0)).astype(np.float32) * np.float32(1.0004)])
     ours = np.array([L.oracle_f32_to_f16(float(v)) for v in x], np.uint16)
-    ref = x.astype(np.float16).view(np.uint16)
+    with np.errstate(over="ignore"):                           # values past 65504 go to inf
+        ref = x.astype(np.float16).view(np.uint16)
     assert np.array_equal(ours, ref)
 
 
