@@ -247,6 +247,38 @@ def test_cli_resumes_reference_format_checkpoint(gpu, tmp_path, monkeypatch):
     assert np.array_equal(a["mean_xz_spanavg"], b["mean_xz_spanavg"], equal_nan=True)
 
 
+def test_cli_pairs_equal_single_steps(gpu, tmp_path, monkeypatch):
+    """The CLI's FP32 default (two steps per launch, the time-mean samples taken inside the
+    pairs at the offsets of each event group) writes the same means, snapshots and Q frames as
+    a run of single steps (--one-step), bit for bit; the force-derived series agree to
+    rounding (the two kernels sum the force partials in different fixed orders)."""
+    from stonewall3d import lbm3d
+    calls = []
+    real = lbm3d.Simulation.step
+
+    def step(self, n=1, accumulate=False, samples=None):
+        calls.append((self.two_step, n, sorted(samples or ())))
+        return real(self, n, accumulate=accumulate, samples=samples)
+
+    monkeypatch.setattr(lbm3d.Simulation, "step", step)
+    pairs, single = str(tmp_path / "pairs"), str(tmp_path / "single")
+    lbm3d.main(["--case", "p", "--out", pairs, "--ckpt-every", "0"] + CLI)
+    sampled = [k for two, n, ks in calls if two for k in ks]
+    # samples at both positions of a pair (sw_step2_accumulate's which = 1 and 2)
+    assert all(two for two, _, _ in calls) and {k % 2 for k in sampled} == {0, 1}
+    calls.clear()
+    lbm3d.main(["--case", "p", "--out", single, "--ckpt-every", "0", "--one-step"] + CLI)
+    assert not any(two for two, _, _ in calls)
+    assert sum(len(ks) for _, _, ks in calls) == len(sampled)
+    a, b = np.load(os.path.join(pairs, "p.npz")), np.load(os.path.join(single, "p.npz"))
+    assert a.files == b.files
+    for k in ("mean_xz_mid", "mean_xy_half", "mean3d_ds2", "snaps_xz", "snaps_xy", "snaps_q", "wall_ds2"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["mean_xz_spanavg"], b["mean_xz_spanavg"], equal_nan=True)
+    for k in ("Cd", "Cl", "Cm", "fx_profile"):
+        assert np.allclose(a[k], b[k], rtol=1e-6, atol=1e-9), k
+
+
 def test_cli_divergence_guard_stops_the_run(gpu, tmp_path, monkeypatch):
     """The run's NaN check (lbm3d.py:386-391, every 5000 steps): a checkpoint whose state
     holds one NaN population in a fluid cell, resumed at step 4999, makes the run stop with
