@@ -24,13 +24,17 @@ Prints ONE JSON line:
                         fraction of it
   fp16_storage          the FP16-storage form timed the same way on the same lattice, with
                         its ratio to the FP32 MLUPS (headline and one-step)
-The comparison legs (copy, one-step FP32, FP16) run before the headline so that it is timed
-at the GPU's settled clocks; the CPU baseline runs last.
+  cd_err_vs_fp64        mean Cd of FP32, FP16 and BF16 storage against the FP64 run on the
+                        40 %-void gusting wall at H = 20 (320 x 160 x 80, 12,000 steps each)
+  serving               p50 / p99 latency and aggregate rate of a short served trace (8
+                        requests, FP32 + FP16, step counts scaled by 0.05)
   cpu_baseline          the FP32 oracle (a C restatement of the reference kernel) on the
                         host: a bounded sample of the same lattice on all cores (value) and
                         on one thread, plus the 160 x 80 x 40 box-wall case
   box_wall_160x80x40_gpu  that box-wall case's 2000 steps on the GPU (a lattice of 0.5 M
                         cells: bound by launch latency and its few tiles, not by HBM)
+The comparison legs (copy, one-step FP32, FP16) run before the headline so that it is timed
+at the GPU's settled clocks; the Cd and serving legs after it; the CPU baseline runs last.
 """
 import argparse
 import json
@@ -162,6 +166,49 @@ def box_wall_gpu(steps=2000, warmup=20):
         raise SystemExit("box-wall force rows are not finite")
     del sim
     return w.cells / (ms * 1e-3) / 1e6, ms
+
+
+def cd_error_leg(H=20, porosity=0.4, seed=0, spin=10.0, avg=20.0):
+    """The metric's Cd error against FP64, on a sample of the precision study's case: the
+    40 %-void gusting wall at H = 20 (320 x 160 x 80), the same seed in every storage form,
+    spin-up and averaging window in H/U as at H = 40 (12,000 steps, one per launch, the
+    time mean sampled every 5 steps).  Mean Cd of FP32, FP16 and BF16 (inlet offsets)
+    relative to the FP64 run's."""
+    from stonewall3d import precision
+    t0 = time.perf_counter()
+    ref = precision.run_case(H, porosity, seed, "f64", spin, avg)
+    out = {"what": f"mean Cd over the averaging window, porous wall H={H} (void {porosity}, seed {seed}, "
+                   f"{spin:g}+{avg:g} H/U = {ref['steps']} steps) against the FP64 run",
+           "cells": ref["cells"], "fp64_mean_cd": round(ref["Cd"], 5)}
+    for d in ("f32", "f16", "bf16"):
+        r = precision.run_case(H, porosity, seed, d, spin, avg, shift="inlet" if d != "f32" else None)
+        if not r["finite"]:
+            raise SystemExit(f"{d} Cd series is not finite")
+        out[d] = {"mean_cd": round(r["Cd"], 5), "rel_err": round((r["Cd"] - ref["Cd"]) / ref["Cd"], 5)}
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
+def serving_leg(n=8, rate=8.0, steps_scale=0.05):
+    """The metric's p50 / p99 case latency on a short served trace: n requests of the serving
+    workload (H ~ U{10..40}, porosity U[0, 0.45], FP32 and FP16, Poisson arrivals at `rate`
+    per second) with their U[2000, 20000] step counts scaled by `steps_scale`, through the
+    continuous-batching server with the library's default tiles (no autotuning)."""
+    import numpy as np
+    from stonewall3d import serving
+    reqs = serving.trace(n, rate, seed=0, steps_scale=steps_scale)
+    srv = serving.Server(budget_bytes=250e9, K=32, policy="sjf", fill_cells=64e6)
+    try:
+        results, rep = srv.serve(reqs)
+    finally:
+        srv.close()
+    if not all(np.isfinite(r.cd_mean) for r in results):
+        raise SystemExit("served Cd is not finite")
+    return {"requests": rep["requests"], "p50_latency_s": round(rep["p50_latency_s"], 3),
+            "p99_latency_s": round(rep["p99_latency_s"], 3), "aggregate_mlups": round(rep["aggregate_mlups"], 1),
+            "utilization": round(rep["utilization"], 4), "wall_s": round(rep["wall_s"], 2),
+            "what": f"{n} requests at {rate:g}/s, steps U[2000, 20000] x {steps_scale:g}, H U{{10..40}}, "
+                    "FP32 + FP16, SJF admission, 32-step chunks"}
 
 
 def copy_ceiling(nbytes, reps=10):
@@ -328,6 +375,8 @@ def main():
                                 "roofline_frac": round(a16 / HBM_PEAK_GBS, 4),
                                 "traffic": tr16, "traffic_source": src16, "storage_offset": kind16}
     if not a.no_cpu_baseline:
+        line["cd_err_vs_fp64"] = cd_error_leg()
+        line["serving"] = serving_leg()
         line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
         # the same box-wall case as the CPU baseline's second sample, on the GPU
         gm, gms = box_wall_gpu()
