@@ -308,6 +308,11 @@ def main():
             legs["f16"] = (t16, k16, s16.shift_kind)
             del s16
             torch.cuda.empty_cache()
+        # The small box-wall case runs the two-step kernel for ~50 ms right before the
+        # headline: after the HBM-bound legs the first launches of the two-step kernel run
+        # up to 8 % slower than its settled rate (launch by launch in the kernel trace), and
+        # the same kernel's load just before the headline shortens that ramp.
+        legs["box_wall"] = box_wall_gpu()
 
     sim, total_ms, kern_ms, per = timed_steps(w, a.dtype, a.shift, a.warmup, a.steps, two_step=two)
     F = sim.force_rows()
@@ -379,7 +384,7 @@ def main():
         line["serving"] = serving_leg()
         line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget)
         # the same box-wall case as the CPU baseline's second sample, on the GPU
-        gm, gms = box_wall_gpu()
+        gm, gms = legs["box_wall"]
         cpu_bw = line["cpu_baseline"]["box_wall_160x80x40"]["all_cores"]["value"]
         line["box_wall_160x80x40_gpu"] = {"value": round(gm, 1), "unit": "MLUPS", "steps": 2000,
                                           "ms_per_step": round(gms, 5), "kernel": "k_step2",
