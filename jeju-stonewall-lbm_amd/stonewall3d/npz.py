@@ -35,6 +35,13 @@ def _dos_time(t):
             (((lt.tm_year - 1980) & 0x7F) << 9) | (lt.tm_mon << 5) | lt.tm_mday)
 
 
+def _numpy_archive(path, arrays):
+    """np.savez_compressed into `path` as given (through a file object, so that numpy does not
+    append ".npz" to it)."""
+    with open(path, "wb") as f:
+        np.savez_compressed(f, **arrays)
+
+
 def savez_compressed(path, workers=None, level=6, block=BLOCK, **arrays):
     """Write `arrays` to `path` (".npz" is not appended) as np.savez_compressed does, with
     the deflate blocks compressed on `workers` threads (default: the CPU count, at most 16)."""
@@ -44,7 +51,7 @@ def savez_compressed(path, workers=None, level=6, block=BLOCK, **arrays):
         np.lib.format.write_array(buf, np.asanyarray(val), allow_pickle=False)
         members.append((f"{key}.npy".encode(), buf.getbuffer()))
     if not members or sum(len(m) for _, m in members) >= _LIMIT:
-        np.savez_compressed(path, **arrays)
+        _numpy_archive(path, arrays)
         return
     workers = workers or min(16, os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=workers) as pool:
@@ -59,7 +66,7 @@ def savez_compressed(path, workers=None, level=6, block=BLOCK, **arrays):
     names = sum(len(name) for name, _ in members)
     end = sum(sizes) + 30 * len(members) + names                       # local headers and data
     if max(sizes) >= _LIMIT or end + 46 * len(members) + names >= _LIMIT or len(members) >= 0xFFFF:
-        np.savez_compressed(path, **arrays)                             # needs ZIP64
+        _numpy_archive(path, arrays)                                    # needs ZIP64
         return
     tm, dt = _dos_time(time.time())
     central = []

@@ -51,3 +51,18 @@ def test_npz_one_block_per_member(tmp_path):
             assert np.array_equal(d[k], v)
     # smooth fields compress
     assert p.stat().st_size < sum(np.asarray(v).nbytes for v in a.values())
+
+
+def test_npz_path_kept_on_every_path(tmp_path):
+    """The archive goes to the path as given, without ".npz" appended: by the threaded writer
+    and by numpy's writer, which takes the inputs the threaded one leaves to it (no arrays)."""
+    a = arrays()
+    p = tmp_path / "fast.out"
+    npz.savez_compressed(str(p), **a)
+    with np.load(p) as d:
+        assert sorted(d.files) == sorted(a)
+    q = tmp_path / "empty.out"
+    npz.savez_compressed(str(q))
+    assert q.exists() and not (tmp_path / "empty.out.npz").exists()
+    with np.load(q) as d:
+        assert d.files == []
